@@ -1,0 +1,153 @@
+/* libheom_amd -- MI355X-native HEOM propagation engine, C ABI.
+ *
+ * This is the drop-in boundary for libheom's time-propagation path.  The
+ * reference exposes it as C++ templates (no C ABI); every entry point below
+ * names the reference interface it replaces.  Conventions:
+ *   - complex numbers are interleaved (re, im) doubles; matrices are N x N
+ *     row-major (order = row_major in the reference);
+ *   - every function returns a status (HEOM_OK == 0); heom_last_error()
+ *     gives the message of the last failure on the calling thread;
+ *   - no C++ exception crosses this boundary;
+ *   - "dev" pointers are HIP device pointers, "host" pointers are host memory.
+ *
+ * The derivative computed everywhere is the reference's
+ *   drho = beta*drho - alpha*D(rho)                 (heom_hilb.h:210-211)
+ * with D the HEOM operator of heom_hilb.h:108-209.
+ */
+#ifndef LIBHEOM_AMD_H
+#define LIBHEOM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HEOM_OK 0
+#define HEOM_ERR_ARG 1            /* bad argument / missing parameter */
+#define HEOM_ERR_STATE 2          /* call out of order (e.g. before set_param) */
+#define HEOM_ERR_HIP 3            /* HIP runtime failure */
+#define HEOM_ERR_STEP_REJECTED 4  /* rkdp gave up after 1000 rejected attempts */
+#define HEOM_ERR_UNSUPPORTED 5    /* size / form combination not provided */
+#define HEOM_ERR_NOMEM 6          /* device allocation failed */
+
+/* QME representation (heom_hilb.h:26, heom_liou.h:27, heom_ado.h:27) */
+#define HEOM_HILB 0
+#define HEOM_LIOU 1
+#define HEOM_ADO 2
+
+/* arithmetic (dtype template parameter: complex128 / complex64) */
+#define HEOM_C128 0  /* FP64 complex everywhere */
+#define HEOM_C64 1   /* FP32 complex operator + stage inputs; FP64 stage
+                        combination, state and error norm */
+
+/* solvers (solver/rk4.h, solver/rkdp.h) */
+#define HEOM_RK4 0
+#define HEOM_RKDP 1
+
+typedef struct heom_qme* heom_handle;
+typedef struct heom_solver* heom_solver_handle;
+
+typedef struct heom_options {
+  int parity_q1;  /* 1 (default): drop off-diagonal gamma like the reference
+                     (heom.h:190-205 clears gamma before copying it) */
+  int sqrt_norm;  /* 1: LIBHEOM_SQRT_NORMALIZATION factors (heom_hilb.h:140-196) */
+  int order;      /* 0: row_major (default), 1: col_major (matrix_base.h) */
+  int device;     /* HIP device ordinal */
+} heom_options;
+
+typedef struct heom_stats {
+  long long n_rhs;          /* calc_diff evaluations (RK stages) */
+  long long n_attempts;     /* rkdp attempts / rk4 steps */
+  long long n_accepted;
+  long long n_rejected;
+  long long ado_stage_updates; /* n_hrchy * n_rhs */
+  int gave_up;              /* rkdp hit the 1000-attempt limit (rkdp.h:95-102) */
+  double last_dt;           /* dt_save after the last solve (adaptive_step_size_solver.h:55) */
+  double last_err;          /* last error norm (rkdp.h:153) */
+} heom_stats;
+
+/* called once per output time t_i, before propagating past it
+ * (solver_base.h:73,78); rho_s is rho[0:N*N] (host, complex interleaved) and
+ * rho_full is the whole host state when the solve copies it, else NULL. */
+typedef void (*heom_callback)(double t, const double* rho_s, const double* rho_full,
+                              void* user);
+
+const char* heom_last_error(void);
+int heom_version(int* major, int* minor);
+int heom_default_options(heom_options* opt);
+
+/* heom_hilb / heom_liou / heom_ado constructor (heom.h:44-50:
+ * heom(int max_depth, int n_inner_threads, int n_outer_threads)); threads are
+ * replaced by the device.  kind: HEOM_HILB|LIOU|ADO, dtype: HEOM_C128|C64. */
+int heom_create(int kind, int dtype, int n_level, int max_depth, const heom_options* opt,
+                heom_handle* out);
+int heom_destroy(heom_handle h);
+
+/* qme_base public members (qme_base.h:32-48) */
+int heom_set_H(heom_handle h, const double* H);
+int heom_alloc_noises(heom_handle h, int n_noise);           /* qme_base.h:54-67 */
+int heom_set_noise(heom_handle h, int u, const double* V, int len_gamma,
+                   const double* gamma, const double* phi_0, const double* sigma,
+                   const double* S, const double* A, double s_delta_re,
+                   double s_delta_im);
+
+/* set_param (heom.h:57-139 + heom_hilb.h:57-68 / heom_liou.h:66-119 /
+ * heom_ado.h:62-189): builds the hierarchy, neighbour links and coefficient
+ * tables and uploads them to HBM. */
+int heom_set_param(heom_handle h);
+int heom_get_n_hrchy(heom_handle h, int* n_hrchy);           /* heom.h:52-55 */
+int heom_get_n_dim(heom_handle h, int* n_dim);
+int heom_main_size(heom_handle h, long long* main_size);     /* heom_hilb.h:45-49 */
+/* hierarchy tables as built (hrchy_space.h:18-27): n (n_hrchy x n_dim),
+ * ptr_p1 / ptr_m1 ((n_hrchy+1) x n_dim, sentinel = n_hrchy).  Any may be NULL. */
+int heom_get_hrchy(heom_handle h, int* n, int* ptr_p1, int* ptr_m1);
+
+/* calc_diff_impl (qme_base.h:74-79): drho_dev = beta*drho_dev - alpha*D(rho_dev)
+ * on device buffers of main_size complex elements of the handle's dtype.
+ * stream: hipStream_t or NULL for the library stream. */
+int heom_calc_diff(heom_handle h, void* drho_dev, const void* rho_dev, double alpha_re,
+                   double alpha_im, double beta_re, double beta_im, void* stream);
+/* same on host buffers (complex128), staging through HBM */
+int heom_calc_diff_host(heom_handle h, double* drho, const double* rho, double alpha_re,
+                        double alpha_im, double beta_re, double beta_im);
+
+/* solver_base::init via qme_solver (qme_solver.h:26-35) */
+int heom_solver_create(heom_handle h, int solver_kind, heom_solver_handle* out);
+int heom_solver_destroy(heom_solver_handle s);
+
+/* qme_solver::solve / solver_base::solve (solver_base.h:64-79): rho_host
+ * (main_size complex128) is propagated in place over t_list; kwargs dt, atol,
+ * rtol as in fixed_step_size_solver.h:37 / adaptive_step_size_solver.h:40-43.
+ * full_copy = 1 copies the whole state to rho_host at every output time
+ * (reference behaviour, solver_base.h:74-76); 0 copies rho_S per output and the
+ * whole state only at the end. */
+int heom_solve(heom_solver_handle s, double* rho_host, const double* t_list, int n_t,
+               heom_callback cb, void* user, double dt, double atol, double rtol,
+               int full_copy);
+/* device-resident variant: rho_dev (main_size, complex128) is propagated in
+ * place; rho_s_out (host, n_t x N*N complex128) receives rho_S at every t_i. */
+int heom_solve_dev(heom_solver_handle s, void* rho_dev, const double* t_list, int n_t,
+                   double dt, double atol, double rtol, double* rho_s_out);
+int heom_solver_stats(heom_solver_handle s, heom_stats* out);
+int heom_solver_reset(heom_solver_handle s); /* forget dt_save (adaptive_step_size_solver.h:28) */
+
+/* measurement helpers: average device duration (ms, HIP events on the
+ * library stream) of each RK stage kernel, replayed `reps` times on the
+ * current state; n_stage receives 4 (rk4) or 7 (rkdp). */
+int heom_solver_time_stages(heom_solver_handle s, int reps, double* ms_per_stage,
+                            int* n_stage);
+/* algorithmic HBM bytes moved by each stage kernel (same order) */
+int heom_solver_stage_bytes(heom_solver_handle s, double* bytes_per_stage, int* n_stage);
+
+/* device memory helpers for callers without their own allocator */
+int heom_dev_alloc(void** ptr, long long bytes);
+int heom_dev_free(void* ptr);
+int heom_memcpy_h2d(void* dst_dev, const void* src_host, long long bytes);
+int heom_memcpy_d2h(void* dst_host, const void* src_dev, long long bytes);
+int heom_device_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBHEOM_AMD_H */

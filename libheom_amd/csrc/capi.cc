@@ -1,0 +1,428 @@
+// extern "C" boundary of libheom_amd (include/libheom_amd.h).  Every entry
+// point catches all exceptions and turns them into status codes.
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/libheom_amd.h"
+#include "kernels.h"
+#include "qme.h"
+#include "solver.h"
+
+struct heom_qme {
+  std::unique_ptr<hamd::Qme> q;
+};
+struct heom_solver {
+  std::unique_ptr<hamd::Solver> s;
+  heom_qme* owner;
+};
+
+namespace hamd {
+namespace {
+thread_local std::string g_err;
+}
+void set_error(int code, const std::string& msg) {
+  (void)code;
+  g_err = msg;
+}
+}  // namespace hamd
+
+namespace {
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    return f();
+  } catch (hamd::HeomError& e) {
+    hamd::set_error(e.code, e.msg);
+    return e.code;
+  } catch (std::exception& e) {
+    hamd::set_error(HEOM_ERR_ARG, e.what());
+    return HEOM_ERR_ARG;
+  } catch (...) {
+    hamd::set_error(HEOM_ERR_ARG, "unknown error");
+    return HEOM_ERR_ARG;
+  }
+}
+
+hamd::Qme& Q(heom_handle h) {
+  if (!h || !h->q) throw hamd::HeomError{HEOM_ERR_ARG, "null handle"};
+  return *h->q;
+}
+
+hamd::Qme& Qready(heom_handle h) {
+  hamd::Qme& q = Q(h);
+  if (!q.ready) throw hamd::HeomError{HEOM_ERR_STATE, "set_param has not been called"};
+  return q;
+}
+
+std::vector<hamd::z128> load_c(const double* p, size_t n) {
+  std::vector<hamd::z128> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = hamd::z128{p[2 * i], p[2 * i + 1]};
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* heom_last_error(void) { return hamd::g_err.c_str(); }
+
+int heom_version(int* major, int* minor) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  return HEOM_OK;
+}
+
+int heom_default_options(heom_options* opt) {
+  if (!opt) return HEOM_ERR_ARG;
+  opt->parity_q1 = 1;
+  opt->sqrt_norm = 0;
+  opt->order = 0;
+  opt->device = 0;
+  return HEOM_OK;
+}
+
+int heom_create(int kind, int dtype, int n_level, int max_depth, const heom_options* opt,
+                heom_handle* out) {
+  return guard([&] {
+    if (!out) throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: out is NULL"};
+    *out = nullptr;
+    if (kind < HEOM_HILB || kind > HEOM_ADO)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: unknown kind"};
+    if (dtype != HEOM_C128 && dtype != HEOM_C64)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: unknown dtype"};
+    if (n_level < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: n_level < 1"};
+    if (max_depth < 0) throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: max_depth < 0"};
+    heom_options o;
+    heom_default_options(&o);
+    if (opt) o = *opt;
+    if (o.order != 0 && o.order != 1)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: order must be 0 or 1"};
+    if (o.order == 1 && kind != HEOM_HILB)
+      throw hamd::HeomError{HEOM_ERR_UNSUPPORTED, "col_major is provided for heom_hilb only"};
+    auto h = std::make_unique<heom_qme>();
+    h->q = std::make_unique<hamd::Qme>(kind, dtype, n_level, max_depth, o);
+    *out = h.release();
+    return HEOM_OK;
+  });
+}
+
+int heom_destroy(heom_handle h) {
+  return guard([&] {
+    delete h;
+    return HEOM_OK;
+  });
+}
+
+int heom_set_H(heom_handle h, const double* H) {
+  return guard([&] {
+    hamd::Qme& q = Q(h);
+    if (!H) throw hamd::HeomError{HEOM_ERR_ARG, "heom_set_H: NULL"};
+    q.H = load_c(H, q.N2);
+    q.H_set = true;
+    q.ready = false;
+    return HEOM_OK;
+  });
+}
+
+int heom_alloc_noises(heom_handle h, int n_noise) {
+  return guard([&] {
+    hamd::Qme& q = Q(h);
+    if (n_noise < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_alloc_noises: n_noise < 1"};
+    q.noise.assign(n_noise, hamd::NoiseParams());
+    q.ready = false;
+    return HEOM_OK;
+  });
+}
+
+int heom_set_noise(heom_handle h, int u, const double* V, int len_gamma, const double* gamma,
+                   const double* phi_0, const double* sigma, const double* S, const double* A,
+                   double s_delta_re, double s_delta_im) {
+  return guard([&] {
+    hamd::Qme& q = Q(h);
+    if (u < 0 || u >= static_cast<int>(q.noise.size()))
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_set_noise: noise index out of range"};
+    if (len_gamma < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_set_noise: len_gamma < 1"};
+    if (!V || !gamma || !phi_0 || !sigma || !S || !A)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_set_noise: NULL parameter"};
+    hamd::NoiseParams& z = q.noise[u];
+    const size_t L = static_cast<size_t>(len_gamma);
+    z.V = load_c(V, q.N2);
+    z.len = len_gamma;
+    z.gamma = load_c(gamma, L * L);
+    z.phi0 = load_c(phi_0, L);
+    z.sigma = load_c(sigma, L);
+    z.S = load_c(S, L * L);
+    z.A = load_c(A, L * L);
+    z.s_delta = hamd::z128{s_delta_re, s_delta_im};
+    z.set = true;
+    q.ready = false;
+    return HEOM_OK;
+  });
+}
+
+int heom_set_param(heom_handle h) {
+  return guard([&] {
+    Q(h).set_param();
+    return HEOM_OK;
+  });
+}
+
+int heom_get_n_hrchy(heom_handle h, int* n) {
+  return guard([&] {
+    if (!n) throw hamd::HeomError{HEOM_ERR_ARG, "NULL output"};
+    *n = Qready(h).hs.n_hrchy;
+    return HEOM_OK;
+  });
+}
+
+int heom_get_n_dim(heom_handle h, int* n) {
+  return guard([&] {
+    if (!n) throw hamd::HeomError{HEOM_ERR_ARG, "NULL output"};
+    *n = Qready(h).hs.n_dim;
+    return HEOM_OK;
+  });
+}
+
+int heom_main_size(heom_handle h, long long* m) {
+  return guard([&] {
+    if (!m) throw hamd::HeomError{HEOM_ERR_ARG, "NULL output"};
+    *m = Qready(h).main_size();
+    return HEOM_OK;
+  });
+}
+
+int heom_get_hrchy(heom_handle h, int* n, int* p1, int* m1) {
+  return guard([&] {
+    hamd::Qme& q = Qready(h);
+    if (n) std::memcpy(n, q.hs.n.data(), sizeof(int) * q.hs.n.size());
+    if (p1) std::memcpy(p1, q.hs.ptr_p1.data(), sizeof(int) * q.hs.ptr_p1.size());
+    if (m1) std::memcpy(m1, q.hs.ptr_m1.data(), sizeof(int) * q.hs.ptr_m1.size());
+    return HEOM_OK;
+  });
+}
+
+int heom_calc_diff(heom_handle h, void* drho_dev, const void* rho_dev, double alpha_re,
+                   double alpha_im, double beta_re, double beta_im, void* stream) {
+  return guard([&] {
+    hamd::Qme& q = Qready(h);
+    if (!drho_dev || !rho_dev) throw hamd::HeomError{HEOM_ERR_ARG, "heom_calc_diff: NULL"};
+    if (drho_dev == rho_dev)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_calc_diff: drho and rho must not alias"};
+    HAMD_HIP(hipSetDevice(q.opt.device));
+    hamd::Slots S{};
+    const int t = q.dtype == HEOM_C64 ? hamd::ST_C64 : hamd::ST_C128;
+    for (int i = 0; i < hamd::kMaxSlots; ++i) S.type[i] = t;
+    S.p[4] = const_cast<void*>(rho_dev);
+    S.p[5] = drho_dev;
+    hamd::StageArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.mode = hamd::MODE_DIFF;
+    A.in_slot = 4;
+    A.out_slot = 5;
+    A.k_slot = -1;
+    A.a_re = alpha_re;
+    A.a_im = alpha_im;
+    A.b_re = beta_re;
+    A.b_im = beta_im;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : q.stream;
+    hamd::launch_operator(q.dev, q.dtype == HEOM_C64 ? 4 : 8, S, A, st);
+    if (!stream) HAMD_HIP(hipStreamSynchronize(st));
+    return HEOM_OK;
+  });
+}
+
+int heom_calc_diff_host(heom_handle h, double* drho, const double* rho, double alpha_re,
+                        double alpha_im, double beta_re, double beta_im) {
+  return guard([&] {
+    hamd::Qme& q = Qready(h);
+    if (!drho || !rho) throw hamd::HeomError{HEOM_ERR_ARG, "heom_calc_diff_host: NULL"};
+    const long long ms = q.main_size();
+    const bool lo = q.dtype == HEOM_C64;
+    const size_t eb = lo ? 8 : 16;
+    hamd::DevBuf din, dout;
+    din.alloc(eb * ms);
+    dout.alloc(eb * ms);
+    if (lo) {
+      std::vector<float> a(2 * ms), b(2 * ms);
+      for (long long i = 0; i < 2 * ms; ++i) {
+        a[i] = static_cast<float>(rho[i]);
+        b[i] = static_cast<float>(drho[i]);
+      }
+      HAMD_HIP(hipMemcpy(din.p, a.data(), eb * ms, hipMemcpyHostToDevice));
+      HAMD_HIP(hipMemcpy(dout.p, b.data(), eb * ms, hipMemcpyHostToDevice));
+    } else {
+      HAMD_HIP(hipMemcpy(din.p, rho, eb * ms, hipMemcpyHostToDevice));
+      HAMD_HIP(hipMemcpy(dout.p, drho, eb * ms, hipMemcpyHostToDevice));
+    }
+    int rc = heom_calc_diff(h, dout.p, din.p, alpha_re, alpha_im, beta_re, beta_im, nullptr);
+    if (rc != HEOM_OK) return rc;
+    if (lo) {
+      std::vector<float> b(2 * ms);
+      HAMD_HIP(hipMemcpy(b.data(), dout.p, eb * ms, hipMemcpyDeviceToHost));
+      for (long long i = 0; i < 2 * ms; ++i) drho[i] = b[i];
+    } else {
+      HAMD_HIP(hipMemcpy(drho, dout.p, eb * ms, hipMemcpyDeviceToHost));
+    }
+    return HEOM_OK;
+  });
+}
+
+int heom_solver_create(heom_handle h, int kind, heom_solver_handle* out) {
+  return guard([&] {
+    if (!out) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solver_create: out is NULL"};
+    *out = nullptr;
+    hamd::Qme& q = Qready(h);
+    auto s = std::make_unique<heom_solver>();
+    s->s = std::make_unique<hamd::Solver>(&q, kind);
+    s->owner = h;
+    *out = s.release();
+    return HEOM_OK;
+  });
+}
+
+int heom_solver_destroy(heom_solver_handle s) {
+  return guard([&] {
+    delete s;
+    return HEOM_OK;
+  });
+}
+
+int heom_solve_dev(heom_solver_handle s, void* rho_dev, const double* t_list, int n_t, double dt,
+                   double atol, double rtol, double* rho_s_out) {
+  return guard([&] {
+    if (!s || !s->s) throw hamd::HeomError{HEOM_ERR_ARG, "null solver"};
+    if (!rho_dev || !t_list) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solve_dev: NULL"};
+    hamd::Solver& sv = *s->s;
+    const int N2 = sv.q->N2;
+    hamd::DevBuf rs;
+    if (rho_s_out) rs.alloc(sizeof(hamd::z128) * N2 * static_cast<size_t>(n_t));
+    auto out = [&](int i, const void* cur) {
+      if (rho_s_out)
+        HAMD_HIP(hipMemcpyAsync(static_cast<char*>(rs.p) + sizeof(hamd::z128) * N2 * i, cur,
+                                sizeof(hamd::z128) * N2, hipMemcpyDeviceToDevice, sv.q->stream));
+    };
+    int rc = sv.solve(rho_dev, t_list, n_t, dt, atol, rtol, out);
+    if (rho_s_out)
+      HAMD_HIP(hipMemcpy(rho_s_out, rs.p, sizeof(hamd::z128) * N2 * static_cast<size_t>(n_t),
+                         hipMemcpyDeviceToHost));
+    if (rc != HEOM_OK)
+      hamd::set_error(rc, "rkdp: step rejected 1000 times; interval skipped as in the reference");
+    return rc;
+  });
+}
+
+int heom_solve(heom_solver_handle s, double* rho_host, const double* t_list, int n_t,
+               heom_callback cb, void* user, double dt, double atol, double rtol,
+               int full_copy) {
+  return guard([&] {
+    if (!s || !s->s) throw hamd::HeomError{HEOM_ERR_ARG, "null solver"};
+    if (!rho_host || !t_list) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solve: NULL"};
+    hamd::Solver& sv = *s->s;
+    const long long ms = sv.q->main_size();
+    const int N2 = sv.q->N2;
+    const size_t bytes = sizeof(hamd::z128) * ms;
+    hamd::DevBuf d;
+    d.alloc(bytes);
+    HAMD_HIP(hipMemcpy(d.p, rho_host, bytes, hipMemcpyHostToDevice));
+    std::vector<double> rs(2 * N2);
+    auto out = [&](int i, const void* cur) {
+      if (full_copy) {
+        HAMD_HIP(hipMemcpyAsync(rho_host, cur, bytes, hipMemcpyDeviceToHost, sv.q->stream));
+        HAMD_HIP(hipStreamSynchronize(sv.q->stream));
+        if (cb) cb(t_list[i], rho_host, rho_host, user);
+      } else {
+        HAMD_HIP(hipMemcpyAsync(rs.data(), cur, sizeof(hamd::z128) * N2, hipMemcpyDeviceToHost,
+                                sv.q->stream));
+        HAMD_HIP(hipStreamSynchronize(sv.q->stream));
+        if (cb) cb(t_list[i], rs.data(), nullptr, user);
+      }
+    };
+    int rc = sv.solve(d.p, t_list, n_t, dt, atol, rtol, out);
+    HAMD_HIP(hipMemcpy(rho_host, d.p, bytes, hipMemcpyDeviceToHost));
+    if (rc != HEOM_OK)
+      hamd::set_error(rc, "rkdp: step rejected 1000 times; interval skipped as in the reference");
+    return rc;
+  });
+}
+
+int heom_solver_stats(heom_solver_handle s, heom_stats* out) {
+  return guard([&] {
+    if (!s || !s->s || !out) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solver_stats: NULL"};
+    *out = s->s->stats;
+    return HEOM_OK;
+  });
+}
+
+int heom_solver_reset(heom_solver_handle s) {
+  return guard([&] {
+    if (!s || !s->s) throw hamd::HeomError{HEOM_ERR_ARG, "null solver"};
+    s->s->dt_save = -1.0;
+    std::memset(&s->s->stats, 0, sizeof(heom_stats));
+    return HEOM_OK;
+  });
+}
+
+int heom_solver_time_stages(heom_solver_handle s, int reps, double* ms_per_stage, int* n_stage) {
+  return guard([&] {
+    if (!s || !s->s || !ms_per_stage) throw hamd::HeomError{HEOM_ERR_ARG, "NULL"};
+    if (reps < 1) throw hamd::HeomError{HEOM_ERR_ARG, "reps < 1"};
+    std::vector<double> v;
+    s->s->time_stages(reps, v);
+    for (size_t i = 0; i < v.size(); ++i) ms_per_stage[i] = v[i];
+    if (n_stage) *n_stage = static_cast<int>(v.size());
+    return HEOM_OK;
+  });
+}
+
+int heom_solver_stage_bytes(heom_solver_handle s, double* bytes, int* n_stage) {
+  return guard([&] {
+    if (!s || !s->s || !bytes) throw hamd::HeomError{HEOM_ERR_ARG, "NULL"};
+    std::vector<double> v;
+    s->s->stage_bytes(v);
+    for (size_t i = 0; i < v.size(); ++i) bytes[i] = v[i];
+    if (n_stage) *n_stage = static_cast<int>(v.size());
+    return HEOM_OK;
+  });
+}
+
+int heom_dev_alloc(void** ptr, long long bytes) {
+  return guard([&] {
+    if (!ptr || bytes < 0) throw hamd::HeomError{HEOM_ERR_ARG, "heom_dev_alloc"};
+    hipError_t e = hipMalloc(ptr, static_cast<size_t>(bytes));
+    if (e != hipSuccess) throw hamd::HeomError{HEOM_ERR_NOMEM, hipGetErrorString(e)};
+    return HEOM_OK;
+  });
+}
+
+int heom_dev_free(void* ptr) {
+  return guard([&] {
+    HAMD_HIP(hipFree(ptr));
+    return HEOM_OK;
+  });
+}
+
+int heom_memcpy_h2d(void* dst, const void* src, long long bytes) {
+  return guard([&] {
+    HAMD_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice));
+    return HEOM_OK;
+  });
+}
+
+int heom_memcpy_d2h(void* dst, const void* src, long long bytes) {
+  return guard([&] {
+    HAMD_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost));
+    return HEOM_OK;
+  });
+}
+
+int heom_device_sync(void) {
+  return guard([&] {
+    HAMD_HIP(hipDeviceSynchronize());
+    return HEOM_OK;
+  });
+}
+
+}  // extern "C"

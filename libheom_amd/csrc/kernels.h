@@ -1,0 +1,24 @@
+// Kernel launchers of the libheom_amd engine.
+#pragma once
+
+#include "common.h"
+
+namespace hamd {
+
+constexpr int kMaxGrid = 8192;  // partial-sum slots for the error norm
+
+// Launch one fused operator pass (calc_diff + epilogue) for the form in `d`.
+// real_bytes: 8 = FP64 operator arithmetic, 4 = FP32.
+// Returns the grid size used (the number of error-norm partials written).
+int launch_operator(const FormDev& d, int real_bytes, const Slots& S, const StageArgs& A,
+                    hipStream_t stream);
+
+// Set up the Dormand-Prince control block for one output interval.
+void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t stream);
+
+// dst (FP32 complex) <- src (FP64 complex), n elements
+void launch_narrow(void* dst, const void* src, long long n, hipStream_t stream);
+// dst <- src for n complex elements of the given slot type
+void launch_copy(void* dst, const void* src, long long n, int type, hipStream_t stream);
+
+}  // namespace hamd

@@ -1,0 +1,106 @@
+// Host-side QME model: parameters, hierarchy, link/coefficient tables and the
+// HBM-resident copies used by the kernels.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/libheom_amd.h"
+#include "common.h"
+
+namespace hamd {
+
+struct NoiseParams {
+  std::vector<z128> V;  // N x N
+  int len = 0;
+  std::vector<z128> gamma, phi0, sigma, S, A;  // len x len / len
+  z128 s_delta{0.0, 0.0};
+  bool set = false;
+};
+
+// Device allocation owned by a QME or a solver.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void alloc(size_t b);
+  template <typename T>
+  void upload(const std::vector<T>& v) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) HAMD_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct Hierarchy {
+  int n_dim = 0, n_hrchy = 0;
+  std::vector<int> n;       // n_hrchy x n_dim
+  std::vector<int> ptr_p1;  // (n_hrchy + 1) x n_dim, sentinel n_hrchy
+  std::vector<int> ptr_m1;
+};
+
+// Breadth-first enumeration with the reference's visiting rule
+// (hrchy_space.cc:137-231): pop an index, then push index + e_k for every
+// k >= the mode last incremented, while the total depth stays <= max_depth.
+Hierarchy build_hierarchy(int n_dim, int max_depth);
+
+class Qme {
+ public:
+  Qme(int kind, int dtype, int n_level, int max_depth, const heom_options& opt);
+  ~Qme();
+
+  int kind, dtype, N, N2, max_depth;
+  heom_options opt;
+  std::vector<z128> H;
+  bool H_set = false;
+  std::vector<NoiseParams> noise;
+  bool ready = false;
+
+  // heom::set_param products
+  Hierarchy hs;
+  std::vector<std::vector<int>> lk;
+  std::vector<z128> ngamma;               // Gamma_n
+  std::vector<std::vector<z128>> s, a;    // per noise, per mode
+  bool diag_v = false;
+
+  // neighbour links
+  std::vector<int> link_ptr;
+  std::vector<Link> links;
+  std::vector<Coef> coef;
+
+  // device copies
+  FormDev dev{};
+  hipStream_t stream = nullptr;
+  DevBuf d_link_ptr, d_links, d_coef, d_gam, d_H, d_V, d_Vd, d_dterm, d_sdelta;
+  DevBuf d_sop_ptr, d_sop_col, d_sop_val;
+  DevBuf d_R_ptr, d_R_col, d_R_val;
+  long long ado_nnz = 0;
+
+  void set_param();
+  long long main_size() const { return static_cast<long long>(hs.n_hrchy) * N2; }
+
+ private:
+  // parameters after the col_major transform (identical for row_major)
+  std::vector<NoiseParams> noise_work_;
+  std::vector<z128> H_work_;
+  std::vector<int> sop_ptr_, sop_col_;
+  std::vector<z128> sop_val_;
+  std::vector<long long> R_ptr_;
+  std::vector<int> R_col_;
+  std::vector<z128> R_val_;
+
+  void build_links();
+  void build_superops(std::vector<int>& ptr, std::vector<int>& col, std::vector<z128>& val,
+                      std::vector<std::vector<std::map<int, z128>>>& ops);
+  void build_ado(const std::vector<std::vector<std::map<int, z128>>>& ops);
+  void upload();
+};
+
+}  // namespace hamd

@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--t-final", type=float, default=1000.0,
+                    help="propagation length in fs (profiling runs use a shorter one)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not provided "
@@ -74,7 +76,7 @@ def main():
     import libheom_amd as H
     from libheom_amd import _capi, configs
 
-    m = configs.exciton_chain7()
+    m = configs.exciton_chain7(t_final=args.t_final)
     eng = H.hip(0)
     qme = H.heom_hilb(m["max_depth"]).load(m, eng)
     solver = H.qme_solver(eng, qme, H.rkdp())
@@ -129,7 +131,7 @@ def main():
         "config": {
             "workload": "7-site exciton chain: N=7, 7 Drude baths x (1 Drude + 2 Pade) = 21 "
                         "modes, depth 4, 12650 ADOs, heom_hilb complex128, Dormand-Prince "
-                        "rtol=1e-8 atol=1e-11, t=0..1000 fs, rho_S every 10 fs",
+                        "rtol=1e-8 atol=1e-11, t=0..%g fs, rho_S every 10 fs" % args.t_final,
             "n_hrchy": qme.n_hrchy, "main_size": qme.main_size(),
             "rhs_evals_per_step": rhs // args.steps,
             "time_to_rho_s_ms": 1e3 * total / args.steps,

@@ -212,23 +212,26 @@ int heom_calc_diff(heom_handle h, void* drho_dev, const void* rho_dev, double al
     if (drho_dev == rho_dev)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_calc_diff: drho and rho must not alias"};
     HAMD_HIP(hipSetDevice(q.opt.device));
-    hamd::Slots S{};
-    const int t = q.dtype == HEOM_C64 ? hamd::ST_C64 : hamd::ST_C128;
-    for (int i = 0; i < hamd::kMaxSlots; ++i) S.type[i] = t;
-    S.p[4] = const_cast<void*>(rho_dev);
-    S.p[5] = drho_dev;
+    const int lo = q.dtype == HEOM_C64 ? 1 : 0;
     hamd::StageArgs A;
     std::memset(&A, 0, sizeof(A));
     A.mode = hamd::MODE_DIFF;
-    A.in_slot = 4;
-    A.out_slot = 5;
-    A.k_slot = -1;
+    A.in[0] = A.in[1] = rho_dev;
+    A.in_lo = lo;
+    A.dst[0][0] = A.dst[0][1] = drho_dev;
+    A.dst_lo[0] = lo;
+    A.ynew_pre = A.yold_pre = -1;
+    if (beta_re != 0.0 || beta_im != 0.0) {
+      A.npre = 1;
+      A.pre[0][0] = A.pre[1][0] = drho_dev;
+      A.pre_lo = lo ? 1u : 0u;
+    }
     A.a_re = alpha_re;
     A.a_im = alpha_im;
     A.b_re = beta_re;
     A.b_im = beta_im;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : q.stream;
-    hamd::launch_operator(q.dev, q.dtype == HEOM_C64 ? 4 : 8, S, A, st);
+    hamd::launch_operator(q.dev, q.dtype == HEOM_C64 ? 4 : 8, A, st);
     if (!stream) HAMD_HIP(hipStreamSynchronize(st));
     return HEOM_OK;
   });

@@ -88,52 +88,57 @@ struct Ctrl {
 };
 
 enum SlotType { ST_C128 = 0, ST_C64 = 1 };
-constexpr int kMaxSlots = 16;
-// Slots 0..3 are ping-pong pairs resolved through Ctrl::cur when a Ctrl is
-// attached: 0 = state (FP64), 1 = new state (FP64), 2 = state (stage-input
-// precision), 3 = new state (stage-input precision).
-struct Slots {
-  void* p[kMaxSlots];
-  int type[kMaxSlots];
-};
-
-// dst (and dst2) <- sum_j c[j]*src[j] + ck*k   (sequential, FP64)
-struct Combo {
-  int nsrc;
-  int src[7];
-  double c[7];
-  double ck;
-  int dst;
-  int dst2;  // -1: none
-};
 
 enum Mode { MODE_DIFF = 0, MODE_STAGE = 1 };
 
+constexpr int kMaxPre = 8;  // state-sized operands an epilogue may read per element
+
+// Epilogue of one operator pass, applied per element after D is known.  All
+// buffer pointers are resolved on the host, once for each value of Ctrl::cur
+// (which buffer of each ping-pong pair holds the current state); without a
+// Ctrl both entries are equal.  "lo" bits mark complex64 operands.
+//   pv[s] = pre[cur][s] at this element (loaded before the operator work)
+// MODE_DIFF : dst[0] = beta*pv[0] - alpha*D      (pv[0] = old drho when beta != 0)
+// MODE_STAGE: k = -dt*D (dt from the control block when attached, else a_re)
+//             k[cur] <- k (optional)
+//             dst[c] (and dst2[c]) <- sum_s cc[c][s]*pv[s] + ck[c]*k, summed in s order
+//             with zero coefficients skipped (so the order follows the reference)
+//   error norm (rkdp.h:142-153): e = sum_s ec[s]*pv[s] + ek*k,
+//             w = |e| / (atol + rtol*max(|pv[ynew]|, |pv[yold]|)); sum of w^2 per workgroup
 struct StageArgs {
   int mode;
-  int in_slot;   // operator input (neighbour reads)
-  // MODE_DIFF: out = beta*out - alpha*D
-  int out_slot;
+  const void* in[2];  // operator input (neighbour reads)
+  int in_lo;
   double a_re, a_im, b_re, b_im;
-  // MODE_STAGE: k = -dt*D (dt from ctrl when attached, else a_re)
-  int k_slot;  // -1: k not stored
+  int npre;
+  const void* pre[2][kMaxPre];
+  unsigned pre_lo;
+  void* k[2];  // null: k not stored
+  int k_lo;
   int ncombo;
-  Combo combo[2];
-  // error norm (rkdp.h:142-153): e = sum ec[j]*k_src[j] + ek*k,
-  // w = |e| / (atol + rtol*max(|y_new|, |y_old|)), partial sums of w^2
+  double cc[2][kMaxPre];
+  double ck[2];
+  void* dst[2][2];   // [combo][cur]
+  void* dst2[2][2];  // null: none
+  int dst_lo[2], dst2_lo[2];
   int do_err;
-  int ne;
-  int esrc[6];
-  double ec[6];
+  double ec[kMaxPre];
   double ek;
-  int ynew_slot, yold_slot;
+  int ynew_pre, yold_pre;
   double atol, rtol;
-  int control;  // 1: last block runs the step control
+  int control;  // 1: the finalize pass runs the step control
   double inv_n;
   Ctrl* ctrl;
   double* partials;
-  unsigned* counter;
   int max_partials;
+};
+
+// One neighbour link with its coefficients inlined (fast Hilbert kernel):
+// contribution cm_u(e)*iphi + ac_u(e)*psi + dir times the neighbour element,
+// with cm = v_i - v_j, ac = v_i + v_j for diagonal V_u (u == n_noise: none).
+struct alignas(16) LinkFlat {
+  int nbr, u, pad0, pad1;
+  z128 iphi, psi, dir;
 };
 
 // Operator tables of one QME in device memory.
@@ -151,6 +156,9 @@ struct FormDev {
   const z128* dterm;     // N2: sum_u delta_u (V_ii - V_jj)^2 (diagonal V)
   const z128* sdelta;    // n_noise
   int diag_v;
+  int real_v;            // diagonal V with real entries
+  int max_links;         // most links of any single ADO
+  const LinkFlat* flat;  // links with inlined coefficients (same order as links)
   // liou: CSR superoperators (N2 x N2): R0, then Phi_u, Psi_u
   const int* sop_ptr;    // (1 + 2*n_noise) * (N2 + 1)
   const int* sop_col;

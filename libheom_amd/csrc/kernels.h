@@ -5,13 +5,16 @@
 
 namespace hamd {
 
-constexpr int kMaxGrid = 8192;  // partial-sum slots for the error norm
+constexpr int kMaxGrid = 1 << 18;  // grid cap = partial-sum slots of the error norm
 
 // Launch one fused operator pass (calc_diff + epilogue) for the form in `d`.
 // real_bytes: 8 = FP64 operator arithmetic, 4 = FP32.
 // Returns the grid size used (the number of error-norm partials written).
-int launch_operator(const FormDev& d, int real_bytes, const Slots& S, const StageArgs& A,
-                    hipStream_t stream);
+int launch_operator(const FormDev& d, int real_bytes, const StageArgs& A, hipStream_t stream);
+
+// Combine the error-norm partials of the last stage pass (nparts = its grid)
+// and run the Dormand-Prince step control.
+void launch_err_finalize(const StageArgs& A, int nparts, hipStream_t stream);
 
 // Set up the Dormand-Prince control block for one output interval.
 void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t stream);

@@ -6,13 +6,13 @@
 //           global-sparse forms, heom_liou.h:151-255, heom_ado.h:191-202)
 //   k     = -dt*D  (calc_diff_impl with alpha = dt, beta = 0; rk4.h:48)
 //   y_i+1 = y + sum_j a_ij k_j  (the next stage input, written in the same pass)
-//   e     = sum_j E_j k_j and the partial sums of the error norm (rkdp.h:142-153)
-// The last workgroup of the final Dormand-Prince stage reduces the partials
-// in a fixed order and runs the step control on the device (rkdp.h:97-177).
+//   e     = sum_j E_j k_j and per-workgroup partial sums of the error norm
+//           (rkdp.h:142-153); err_finalize_kernel reduces them in a fixed
+//           order and runs the step control on the device (rkdp.h:97-177).
 //
-// Threads map to matrix elements: thread (a, e) of a workgroup handles element
-// e = i*N + j of the a-th ADO of the group, so every ADO block (row-major
-// N x N, the reference layout) is read and written with contiguous lanes.
+// Threads map to matrix elements: thread e of an ADO's lane group handles
+// element e = i*N + j, so every ADO block (row-major N x N, the reference
+// layout) is read and written by contiguous lanes.
 #include <hip/hip_runtime.h>
 
 #include "../../include/libheom_amd.h"
@@ -23,19 +23,34 @@ namespace hamd {
 
 namespace {
 
-__device__ __forceinline__ void* slot_ptr(const Slots& S, int s, int cur) {
-  if (s < 4) return S.p[(s & 2) + ((s & 1) ^ cur)];
-  return S.p[s];
+// read-only tables through the constant address space: uniform indices then
+// compile to scalar loads
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* cst(const T* p) {
+  return (const __attribute__((address_space(4))) T*)(p);
 }
 
-__device__ __forceinline__ z128 ld(const void* p, int type, long long i) {
-  if (type == ST_C128) return static_cast<const z128*>(p)[i];
-  z64 v = static_cast<const z64*>(p)[i];
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
+__device__ __forceinline__ LinkFlat ld_flat(cptr<LinkFlat> p) {
+  LinkFlat f;
+  f.nbr = p->nbr;
+  f.u = p->u;
+  f.iphi = z128{p->iphi.re, p->iphi.im};
+  f.psi = z128{p->psi.re, p->psi.im};
+  f.dir = z128{p->dir.re, p->dir.im};
+  return f;
+}
+
+__device__ __forceinline__ z128 ldz(const void* p, bool lo, long long i) {
+  if (!lo) return static_cast<const z128*>(p)[i];
+  const z64 v = static_cast<const z64*>(p)[i];
   return z128{static_cast<double>(v.re), static_cast<double>(v.im)};
 }
 
-__device__ __forceinline__ void st(void* p, int type, long long i, z128 v) {
-  if (type == ST_C128)
+__device__ __forceinline__ void stz(void* p, bool lo, long long i, z128 v) {
+  if (!lo)
     static_cast<z128*>(p)[i] = v;
   else
     static_cast<z64*>(p)[i] = z64{static_cast<float>(v.re), static_cast<float>(v.im)};
@@ -43,59 +58,80 @@ __device__ __forceinline__ void st(void* p, int type, long long i, z128 v) {
 
 __device__ __forceinline__ double zabs(z128 a) { return sqrt(a.re * a.re + a.im * a.im); }
 
+// parity select without indexing kernel-argument arrays by a runtime value
+// (which would copy the arguments to scratch memory)
+template <typename T>
+__device__ __forceinline__ T pick(const T (&a)[2], int cur) {
+  return cur ? a[1] : a[0];
+}
+
+// Per-element epilogue operands, loaded before the operator work so that
+// their memory latency overlaps it.  F64: every operand is complex128.
+struct Pre {
+  z128 v[kMaxPre];
+};
+
+template <bool F64>
+__device__ __forceinline__ void prefetch(Pre& p, const StageArgs& A, int cur, long long gi,
+                                         bool ok) {
+#pragma unroll
+  for (int s = 0; s < kMaxPre; ++s) {
+    const void* ptr = cur ? A.pre[1][s] : A.pre[0][s];
+    if (ok && s < A.npre && ptr != nullptr)
+      p.v[s] = ldz(ptr, F64 ? false : ((A.pre_lo >> s) & 1u), gi);
+    else
+      p.v[s] = z128{0.0, 0.0};
+  }
+}
+
+// Linear combination sum_s c[s]*p[s] + ck*k, accumulated in s order.  Terms
+// with a zero coefficient add an exact zero, so the result equals the
+// sequential sum over the non-zero terms (reference order).
+__device__ __forceinline__ z128 combine(const Pre& p, const double* c, int npre, double ck,
+                                        z128 k) {
+  z128 v{0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < kMaxPre; ++s) {
+    if (s < npre) {
+      v.re = fma(c[s], p.v[s].re, v.re);
+      v.im = fma(c[s], p.v[s].im, v.im);
+    }
+  }
+  v.re = fma(ck, k.re, v.re);
+  v.im = fma(ck, k.im, v.im);
+  return v;
+}
+
 // Epilogue shared by all operator kernels.  D is the operator applied to the
 // stage input at flat element gi; returns w^2 of the error norm (or 0).
-template <typename R>
-__device__ __forceinline__ double epilogue(const StageArgs& A, const Slots& S, int cur,
+template <bool F64, typename R>
+__device__ __forceinline__ double epilogue(const Pre& p, const StageArgs& A, int cur,
                                            long long gi, cx<R> Dr, z128 alpha) {
-  z128 D = conv<double>(Dr);
-  // -alpha*D
-  z128 k = z128{-(alpha.re * D.re - alpha.im * D.im), -(alpha.re * D.im + alpha.im * D.re)};
+  const z128 D = conv<double>(Dr);
+  const z128 k{-(alpha.re * D.re - alpha.im * D.im), -(alpha.re * D.im + alpha.im * D.re)};
   if (A.mode == MODE_DIFF) {
-    const int o = A.out_slot;
-    void* op = slot_ptr(S, o, cur);
     z128 v = k;
-    if (A.b_re != 0.0 || A.b_im != 0.0) {
-      z128 old = ld(op, S.type[o], gi);
-      v = z128{A.b_re * old.re - A.b_im * old.im + k.re, A.b_re * old.im + A.b_im * old.re + k.im};
+    if (A.npre > 0) {
+      const z128 o = p.v[0];
+      v = z128{A.b_re * o.re - A.b_im * o.im + k.re, A.b_re * o.im + A.b_im * o.re + k.im};
     }
-    st(op, S.type[o], gi, v);
+    stz(pick(A.dst[0], cur), F64 ? false : A.dst_lo[0], gi, v);
     return 0.0;
   }
-  if (A.k_slot >= 0) st(slot_ptr(S, A.k_slot, cur), S.type[A.k_slot], gi, k);
-  for (int c = 0; c < A.ncombo; ++c) {
-    const Combo& cb = A.combo[c];
-    z128 v{0.0, 0.0};
-    for (int s = 0; s < cb.nsrc; ++s) {
-      z128 x = ld(slot_ptr(S, cb.src[s], cur), S.type[cb.src[s]], gi);
-      if (s == 0) {
-        v = cb.c[0] * x;
-      } else {
-        v.re += cb.c[s] * x.re;
-        v.im += cb.c[s] * x.im;
-      }
+  if (void* kp = pick(A.k, cur)) stz(kp, F64 ? false : A.k_lo, gi, k);
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (c < A.ncombo) {
+      const z128 v = combine(p, A.cc[c], A.npre, A.ck[c], k);
+      stz(pick(A.dst[c], cur), F64 ? false : A.dst_lo[c], gi, v);
+      if (void* d2 = pick(A.dst2[c], cur)) stz(d2, F64 ? false : A.dst2_lo[c], gi, v);
     }
-    v.re += cb.ck * k.re;
-    v.im += cb.ck * k.im;
-    st(slot_ptr(S, cb.dst, cur), S.type[cb.dst], gi, v);
-    if (cb.dst2 >= 0) st(slot_ptr(S, cb.dst2, cur), S.type[cb.dst2], gi, v);
   }
   if (A.do_err) {
-    z128 e{0.0, 0.0};
-    for (int s = 0; s < A.ne; ++s) {
-      z128 x = ld(slot_ptr(S, A.esrc[s], cur), S.type[A.esrc[s]], gi);
-      if (s == 0) {
-        e = A.ec[0] * x;
-      } else {
-        e.re += A.ec[s] * x.re;
-        e.im += A.ec[s] * x.im;
-      }
-    }
-    e.re += A.ek * k.re;
-    e.im += A.ek * k.im;
-    double y1 = zabs(ld(slot_ptr(S, A.ynew_slot, cur), S.type[A.ynew_slot], gi));
-    double y0 = zabs(ld(slot_ptr(S, A.yold_slot, cur), S.type[A.yold_slot], gi));
-    double w = zabs(e) / (A.atol + A.rtol * (y1 > y0 ? y1 : y0));
+    // operands kMaxPre-2 / kMaxPre-1 hold y_new / y_old (zero weights in ec)
+    const z128 e = combine(p, A.ec, kMaxPre - 2, A.ek, k);
+    const double a1 = zabs(p.v[kMaxPre - 2]), a0 = zabs(p.v[kMaxPre - 1]);
+    const double w = zabs(e) / (A.atol + A.rtol * (a1 > a0 ? a1 : a0));
     return w * w;
   }
   return 0.0;
@@ -139,11 +175,10 @@ __device__ void dopri_control(Ctrl* c, double err) {
   c->dt = dt;
 }
 
-// Block reduction of the error-norm partials; the last block to finish sums
-// all partials in block order (deterministic) and runs the control.
-__device__ void finish_error(const StageArgs& A, double v) {
+// Block reduction of the error-norm partial sums: one value per workgroup,
+// written without atomics; err_finalize_kernel combines them.
+__device__ void write_partial(const StageArgs& A, double v) {
   __shared__ double s_red[16];
-  __shared__ int s_last;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nw = (blockDim.x + 63) >> 6;
@@ -152,73 +187,223 @@ __device__ void finish_error(const StageArgs& A, double v) {
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < nw; ++w) t += s_red[w];
-    __hip_atomic_store(&A.partials[blockIdx.x], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned ticket = atomicAdd(A.counter, 1u);
-    s_last = (ticket == gridDim.x - 1) ? 1 : 0;
+    A.partials[blockIdx.x] = t;
   }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  double x = 0.0;
-  for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x)
-    x += __hip_atomic_load(&A.partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sums the partials of the last stage pass in a fixed order (deterministic)
+// and runs the step control (one workgroup of 1024 threads; every thread sums
+// a strided slice with four independent accumulators).
+__global__ void __launch_bounds__(1024) err_finalize_kernel(const StageArgs A, int nparts) {
+  __shared__ double s_red[16];
+  if (A.ctrl && A.ctrl->done) return;
+  double x0 = 0.0, x1 = 0.0, x2 = 0.0, x3 = 0.0;
+  int b = threadIdx.x;
+  for (; b + 3 * 1024 < nparts; b += 4 * 1024) {
+    x0 += A.partials[b];
+    x1 += A.partials[b + 1024];
+    x2 += A.partials[b + 2048];
+    x3 += A.partials[b + 3072];
+  }
+  for (; b < nparts; b += 1024) x0 += A.partials[b];
+  double x = (x0 + x1) + (x2 + x3);
   for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-  __syncthreads();
-  if (lane == 0) s_red[wid] = x;
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
-    for (int w = 0; w < nw; ++w) t += s_red[w];
+    for (int w = 0; w < 16; ++w) t += s_red[w];
     double err = sqrt(t * A.inv_n);
     if (A.control)
       dopri_control(A.ctrl, err);
     else if (A.ctrl)
       A.ctrl->err = err;
-    __hip_atomic_store(A.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
+// Kernel prologue: leave early when the control block says the interval is
+// done, pick the ping-pong parity and the step.
+struct Prologue {
+  int cur;
+  z128 alpha;
+  bool skip;
+};
+
+__device__ __forceinline__ Prologue prologue(const StageArgs& A) {
+  Prologue p{0, z128{A.a_re, A.a_im}, false};
+  if (A.ctrl) {
+    if (A.ctrl->done) {
+      p.skip = true;
+      return p;
+    }
+    p.cur = A.ctrl->cur;
+    if (A.mode == MODE_STAGE) p.alpha = z128{A.ctrl->dt, 0.0};
+  }
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// heom_hilb, fast path: one ADO per workgroup (N = 6..16), diagonal V
+// (heom_hilb.h:96-212 with [V,X]_ij = (v_i - v_j) X_ij, {V,X}_ij = (v_i + v_j) X_ij)
+// ---------------------------------------------------------------------------
+template <int N>
+struct WaveCfg {
+  static constexpr int N2 = N * N;
+  static constexpr int BS = ((N2 + 63) / 64) * 64;
+};
+
+// cm/ac table in LDS: per noise u and element e the pair (v_i - v_j, v_i + v_j)
+// (real V: 2 reals; complex V: 2 complex), plus one zero row for direct links.
+template <typename R, bool REALV>
+size_t fast_lds_bytes(int n_noise, int N2) {
+  return sizeof(R) * (REALV ? 2 : 4) * static_cast<size_t>(n_noise + 1) * N2;
+}
+
+// coefficient of one link at element `base` = u*N2 + e of the cm/ac table
+template <typename R, bool REALV>
+__device__ __forceinline__ cx<R> fast_coef(const R* s_cm, const LinkFlat& q, int base) {
+  cx<R> cf;
+  if (REALV) {
+    const R cm = s_cm[2 * base], ac = s_cm[2 * base + 1];
+    cf.re = cm * static_cast<R>(q.iphi.re) + ac * static_cast<R>(q.psi.re) +
+            static_cast<R>(q.dir.re);
+    cf.im = cm * static_cast<R>(q.iphi.im) + ac * static_cast<R>(q.psi.im) +
+            static_cast<R>(q.dir.im);
+  } else {
+    const cx<R> cm{s_cm[4 * base], s_cm[4 * base + 1]};
+    const cx<R> ac{s_cm[4 * base + 2], s_cm[4 * base + 3]};
+    cf = cm * conv<R>(q.iphi) + ac * conv<R>(q.psi) + conv<R>(q.dir);
+  }
+  return cf;
+}
+
+template <int N, typename R, bool REALV, bool F64>
+__global__ void __launch_bounds__(WaveCfg<N>::BS)
+    hilb_fast_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  constexpr int N2 = N * N, BS = WaveCfg<N>::BS;
+  __shared__ C s_rho[N2];
+  __shared__ C s_H[N2];
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  R* s_cm = reinterpret_cast<R*>(s_dyn);
+
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const int e = threadIdx.x;
+  const bool lane_ok = e < N2;
+  const int ee = lane_ok ? e : 0;
+  const int i = ee / N, j = ee - (ee / N) * N;
+  const int nn = d.n_noise;
+  for (int x = e; x < N2; x += BS) s_H[x] = conv<R>(d.H[x]);
+  for (int x = e; x < (nn + 1) * N2; x += BS) {
+    const int u = x / N2, y = x - u * N2;
+    const int ii = y / N, jj = y - (y / N) * N;
+    if (REALV) {
+      R cm = 0, ac = 0;
+      if (u < nn) {
+        cm = static_cast<R>(d.Vd[u * N + ii].re - d.Vd[u * N + jj].re);
+        ac = static_cast<R>(d.Vd[u * N + ii].re + d.Vd[u * N + jj].re);
+      }
+      s_cm[2 * x] = cm;
+      s_cm[2 * x + 1] = ac;
+    } else {
+      C cm{0, 0}, ac{0, 0};
+      if (u < nn) {
+        cm = conv<R>(d.Vd[u * N + ii] - d.Vd[u * N + jj]);
+        ac = conv<R>(d.Vd[u * N + ii] + d.Vd[u * N + jj]);
+      }
+      s_cm[4 * x] = cm.re;
+      s_cm[4 * x + 1] = cm.im;
+      s_cm[4 * x + 2] = ac.re;
+      s_cm[4 * x + 3] = ac.im;
+    }
+  }
+  const C dterm = lane_ok ? conv<R>(d.dterm[ee]) : C{0, 0};
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  const auto link_ptr = cst(d.link_ptr);
+  const auto flat = cst(d.flat);
+  const auto gam = cst(d.gam);
+
+  // one ADO per workgroup; idle lanes (e >= N2) read element 0 and store nothing
+  const int g = blockIdx.x;
+  const long long gi = static_cast<long long>(g) * N2 + ee;
+  const C r = in[gi];
+  Pre pv;
+  prefetch<F64>(pv, A, cur, gi, true);
+  const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
+  const C gm = conv<R>(z128{gam[g].re, gam[g].im});
+  if (lane_ok) s_rho[e] = r;
+  __syncthreads();
+  C acc{0, 0};
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    cmac(acc, s_H[i * N + k], s_rho[k * N + j]);
+    const C t = s_rho[i * N + k];
+    cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
+  }
+  C D = times_i(acc);
+  cmac(D, gm + dterm, r);
+  int l = l0;
+  for (; l + 4 <= l1; l += 4) {
+    LinkFlat q[4];
+    C x[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) q[t] = ld_flat(flat + l + t);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) x[t] = in[static_cast<long long>(q[t].nbr) * N2 + ee];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) cmac(D, fast_coef<R, REALV>(s_cm, q[t], q[t].u * N2 + ee), x[t]);
+  }
+  for (; l < l1; ++l) {
+    const LinkFlat q = ld_flat(flat + l);
+    const C x = in[static_cast<long long>(q.nbr) * N2 + ee];
+    cmac(D, fast_coef<R, REALV>(s_cm, q, q.u * N2 + ee), x);
+  }
+  double w2 = 0.0;
+  if (lane_ok) w2 = epilogue<F64, R>(pv, A, cur, gi, D, pg.alpha);
+  if (A.do_err) write_partial(A, w2);
+}
+
+// ---------------------------------------------------------------------------
+// heom_hilb, general path: any V, N <= 16; N <= 8 packs 64/N^2 ADOs per
+// wavefront (heom_hilb.h:96-212)
+// ---------------------------------------------------------------------------
 template <int N>
 struct HilbCfg {
   static constexpr int N2 = N * N;
-  static constexpr int BS = N2 <= 64 ? 256 : ((N2 + 63) / 64) * 64;
+  static constexpr int BS = N2 <= 64 ? 64 : ((N2 + 63) / 64) * 64;
   static constexpr int APB = BS / N2;  // ADOs per workgroup
 };
 
-// ---------------------------------------------------------------------------
-// heom_hilb: per-ADO commutator form (heom_hilb.h:96-212), one element per lane
-// ---------------------------------------------------------------------------
-template <int N, typename R, bool DIAGV>
+template <int N, typename R, bool DIAGV, bool F64>
 __global__ void __launch_bounds__(HilbCfg<N>::BS)
-    hilb_kernel(const FormDev d, const Slots S, const StageArgs A) {
+    hilb_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
-  constexpr int N2 = N * N, APB = HilbCfg<N>::APB;
+  constexpr int N2 = N * N, APB = HilbCfg<N>::APB, BS = HilbCfg<N>::BS;
   __shared__ C s_rho[APB * N2];
+  __shared__ C s_H[N2];
   __shared__ C s_X[DIAGV ? 1 : APB * N2];
   __shared__ C s_Y[DIAGV ? 1 : APB * N2];
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  C* s_vd = reinterpret_cast<C*>(s_dyn);
 
-  int cur = 0;
-  z128 alpha{A.a_re, A.a_im};
-  if (A.ctrl) {
-    if (A.ctrl->done) return;
-    cur = A.ctrl->cur;
-    if (A.mode == MODE_STAGE) alpha = z128{A.ctrl->dt, 0.0};
-  }
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
   const int tid = threadIdx.x;
   const int la = tid / N2;
   const int e = tid - la * N2;
   const int i = e / N, j = e - (e / N) * N;
   const bool lane_ok = la < APB;
-
-  C Hr[N], Hc[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    Hr[k] = lane_ok ? conv<R>(d.H[i * N + k]) : C{0, 0};
-    Hc[k] = lane_ok ? conv<R>(d.H[k * N + j]) : C{0, 0};
+  for (int x = tid; x < N2; x += BS) s_H[x] = conv<R>(d.H[x]);
+  if (DIAGV) {
+    // diagonal of every V_u, then one zero row for direct (off-diagonal gamma) links
+    for (int x = tid; x < (d.n_noise + 1) * N; x += BS)
+      s_vd[x] = x < d.n_noise * N ? conv<R>(d.Vd[x]) : C{0, 0};
   }
   const C dterm = (DIAGV && lane_ok) ? conv<R>(d.dterm[e]) : C{0, 0};
-  const C* in = static_cast<const C*>(slot_ptr(S, A.in_slot, cur));
+  const C* in = static_cast<const C*>(pick(A.in, cur));
 
   double err_acc = 0.0;
   const int n_groups = (d.n_hrchy + APB - 1) / APB;
@@ -227,6 +412,10 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
     const bool ok = lane_ok && n < d.n_hrchy;
     const long long gi = static_cast<long long>(n) * N2 + e;
     const C r = ok ? in[gi] : C{0, 0};
+    Pre pv;
+    prefetch<F64>(pv, A, cur, gi, ok);
+    const int l0 = ok ? d.link_ptr[n] : 0;
+    const int l1 = ok ? d.link_ptr[n + 1] : 0;
     if (lane_ok) s_rho[tid] = r;
     __syncthreads();
     C acc{0, 0};
@@ -234,32 +423,35 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
       const C* b = s_rho + la * N2;
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        cmac(acc, Hr[k], b[k * N + j]);
-        cmac(acc, C{-b[i * N + k].re, -b[i * N + k].im}, Hc[k]);
+        cmac(acc, s_H[i * N + k], b[k * N + j]);
+        const C t = b[i * N + k];
+        cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
       }
     }
     C D = times_i(acc);
-    int l0 = 0, l1 = 0;
-    if (ok) {
-      const C gm = conv<R>(d.gam[n]);
-      cmac(D, gm + dterm, r);
-      l0 = d.link_ptr[n];
-      l1 = d.link_ptr[n + 1];
-    }
+    if (ok) cmac(D, conv<R>(d.gam[n]) + dterm, r);
     if (DIAGV) {
-      for (int l = l0; l < l1; ++l) {
-        const Link lk = d.links[l];
-        const Coef c = d.coef[lk.cid];
-        const C x = in[static_cast<long long>(lk.nbr) * N2 + e];
-        C cf = conv<R>(c.dir);
-        if (c.u < d.n_noise) {
-          const C vi = conv<R>(d.Vd[c.u * N + i]);
-          const C vj = conv<R>(d.Vd[c.u * N + j]);
-          cf = cf + times_i(vi - vj) * conv<R>(c.phi) + (vi + vj) * conv<R>(c.psi);
-        }
-        cmac(D, cf, x);
+      int l = l0;
+      for (; l + 2 <= l1; l += 2) {
+        const LinkFlat q0 = d.flat[l], q1 = d.flat[l + 1];
+        const C x0 = in[static_cast<long long>(q0.nbr) * N2 + e];
+        const C x1 = in[static_cast<long long>(q1.nbr) * N2 + e];
+        const C vi0 = s_vd[q0.u * N + i], vj0 = s_vd[q0.u * N + j];
+        const C vi1 = s_vd[q1.u * N + i], vj1 = s_vd[q1.u * N + j];
+        cmac(D, (vi0 - vj0) * conv<R>(q0.iphi) + (vi0 + vj0) * conv<R>(q0.psi) + conv<R>(q0.dir),
+             x0);
+        cmac(D, (vi1 - vj1) * conv<R>(q1.iphi) + (vi1 + vj1) * conv<R>(q1.psi) + conv<R>(q1.dir),
+             x1);
+      }
+      for (; l < l1; ++l) {
+        const LinkFlat q0 = d.flat[l];
+        const C x0 = in[static_cast<long long>(q0.nbr) * N2 + e];
+        const C vi0 = s_vd[q0.u * N + i], vj0 = s_vd[q0.u * N + j];
+        cmac(D, (vi0 - vj0) * conv<R>(q0.iphi) + (vi0 + vj0) * conv<R>(q0.psi) + conv<R>(q0.dir),
+             x0);
       }
     } else {
+      // dense V: Phi_u, Psi_u per noise, then V_u X + Y V_u (heom_hilb.h:113-208)
       int l = l0;
       for (int u = 0; u < d.n_noise; ++u) {
         C phi{0, 0}, psi{0, 0};
@@ -311,36 +503,31 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
         cmac(D, conv<R>(c.dir), in[static_cast<long long>(lk.nbr) * N2 + e]);
       }
     }
-    if (ok) err_acc += epilogue<R>(A, S, cur, gi, D, alpha);
+    if (ok) err_acc += epilogue<F64, R>(pv, A, cur, gi, D, pg.alpha);
     __syncthreads();
   }
-  if (A.do_err) finish_error(A, err_acc);
+  if (A.do_err) write_partial(A, err_acc);
 }
 
 // ---------------------------------------------------------------------------
 // heom_liou: per-ADO superoperator form (heom_liou.h:151-255), N <= 16
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ void __launch_bounds__(256)
-    liou_kernel(const FormDev d, const Slots S, const StageArgs A) {
+template <typename R, bool F64>
+__global__ void __launch_bounds__(256) liou_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   __shared__ C s_rho[256];
   __shared__ C s_phi[256];
   __shared__ C s_psi[256];
-  int cur = 0;
-  z128 alpha{A.a_re, A.a_im};
-  if (A.ctrl) {
-    if (A.ctrl->done) return;
-    cur = A.ctrl->cur;
-    if (A.mode == MODE_STAGE) alpha = z128{A.ctrl->dt, 0.0};
-  }
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
   const int N2 = d.N2;
   const int APB = 256 / N2;
   const int tid = threadIdx.x;
   const int la = tid / N2;
   const int e = tid - la * N2;
   const bool lane_ok = la < APB;
-  const C* in = static_cast<const C*>(slot_ptr(S, A.in_slot, cur));
+  const C* in = static_cast<const C*>(pick(A.in, cur));
   const int stride = N2 + 1;
   double err_acc = 0.0;
   const int n_groups = (d.n_hrchy + APB - 1) / APB;
@@ -349,6 +536,8 @@ __global__ void __launch_bounds__(256)
     const bool ok = lane_ok && n < d.n_hrchy;
     const long long gi = static_cast<long long>(n) * N2 + e;
     const C r = ok ? in[gi] : C{0, 0};
+    Pre pv;
+    prefetch<F64>(pv, A, cur, gi, ok);
     if (lane_ok) s_rho[tid] = r;
     __syncthreads();
     C D{0, 0};
@@ -392,37 +581,34 @@ __global__ void __launch_bounds__(256)
       const Coef c = d.coef[lk.cid];
       cmac(D, conv<R>(c.dir), in[static_cast<long long>(lk.nbr) * N2 + e]);
     }
-    if (ok) err_acc += epilogue<R>(A, S, cur, gi, D, alpha);
+    if (ok) err_acc += epilogue<F64, R>(pv, A, cur, gi, D, pg.alpha);
     __syncthreads();
   }
-  if (A.do_err) finish_error(A, err_acc);
+  if (A.do_err) write_partial(A, err_acc);
 }
 
 // ---------------------------------------------------------------------------
 // heom_ado: one global CSR matrix-vector product (heom_ado.h:191-202)
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ void __launch_bounds__(256)
-    ado_kernel(const FormDev d, const Slots S, const StageArgs A) {
+template <typename R, bool F64>
+__global__ void __launch_bounds__(256) ado_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
-  int cur = 0;
-  z128 alpha{A.a_re, A.a_im};
-  if (A.ctrl) {
-    if (A.ctrl->done) return;
-    cur = A.ctrl->cur;
-    if (A.mode == MODE_STAGE) alpha = z128{A.ctrl->dt, 0.0};
-  }
-  const C* in = static_cast<const C*>(slot_ptr(S, A.in_slot, cur));
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
   double err_acc = 0.0;
   const long long rows = d.main_size;
   for (long long r = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; r < rows;
        r += static_cast<long long>(gridDim.x) * blockDim.x) {
+    Pre pv;
+    prefetch<F64>(pv, A, cur, r, true);
     C D{0, 0};
     for (long long p = d.R_ptr[r]; p < d.R_ptr[r + 1]; ++p)
       cmac(D, conv<R>(d.R_val[p]), in[d.R_col[p]]);
-    err_acc += epilogue<R>(A, S, cur, r, D, alpha);
+    err_acc += epilogue<F64, R>(pv, A, cur, r, D, pg.alpha);
   }
-  if (A.do_err) finish_error(A, err_acc);
+  if (A.do_err) write_partial(A, err_acc);
 }
 
 __global__ void ctrl_init_kernel(Ctrl* c, double t0, double t_bound, double dt) {
@@ -450,25 +636,51 @@ __global__ void copy_kernel(T* dst, const T* src, long long n) {
     dst[i] = src[i];
 }
 
-template <int N, typename R>
-int launch_hilb_n(const FormDev& d, const Slots& S, const StageArgs& A, hipStream_t st) {
+inline int cap_grid(long long n) {
+  if (n < 1) return 1;
+  return n < kMaxGrid ? static_cast<int>(n) : kMaxGrid;
+}
+
+template <int N, typename R, bool F64>
+int launch_hilb_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  if (N >= 6 && d.diag_v && d.n_hrchy <= kMaxGrid) {
+    constexpr int BS = WaveCfg<N>::BS;
+    const int grid = d.n_hrchy;
+    if (d.real_v) {
+      const size_t lds = fast_lds_bytes<R, true>(d.n_noise, N * N);
+      if (lds <= 48 * 1024) {
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64>), dim3(grid), dim3(BS), lds, st, d,
+                           A);
+        return grid;
+      }
+    } else {
+      const size_t lds = fast_lds_bytes<R, false>(d.n_noise, N * N);
+      if (lds <= 48 * 1024) {
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, false, F64>), dim3(grid), dim3(BS), lds, st,
+                           d, A);
+        return grid;
+      }
+    }
+  }
   constexpr int APB = HilbCfg<N>::APB, BS = HilbCfg<N>::BS;
-  int groups = (d.n_hrchy + APB - 1) / APB;
-  int grid = groups < kMaxGrid ? groups : kMaxGrid;
-  if (grid < 1) grid = 1;
-  if (d.diag_v)
-    hipLaunchKernelGGL((hilb_kernel<N, R, true>), dim3(grid), dim3(BS), 0, st, d, S, A);
-  else
-    hipLaunchKernelGGL((hilb_kernel<N, R, false>), dim3(grid), dim3(BS), 0, st, d, S, A);
+  const int grid = cap_grid((d.n_hrchy + APB - 1) / APB);
+  if (d.diag_v) {
+    const size_t lds = sizeof(cx<R>) * (d.n_noise + 1) * N;
+    if (lds > 48 * 1024)
+      throw HeomError{HEOM_ERR_UNSUPPORTED, "heom_hilb: too many noises for the LDS tables"};
+    hipLaunchKernelGGL((hilb_kernel<N, R, true, F64>), dim3(grid), dim3(BS), lds, st, d, A);
+  } else {
+    hipLaunchKernelGGL((hilb_kernel<N, R, false, F64>), dim3(grid), dim3(BS), 0, st, d, A);
+  }
   return grid;
 }
 
-template <typename R>
-int launch_hilb(const FormDev& d, const Slots& S, const StageArgs& A, hipStream_t st) {
+template <typename R, bool F64>
+int launch_hilb(const FormDev& d, const StageArgs& A, hipStream_t st) {
   switch (d.N) {
 #define HAMD_CASE(n) \
   case n:            \
-    return launch_hilb_n<n, R>(d, S, A, st);
+    return launch_hilb_n<n, R, F64>(d, A, st);
     HAMD_CASE(1) HAMD_CASE(2) HAMD_CASE(3) HAMD_CASE(4) HAMD_CASE(5) HAMD_CASE(6) HAMD_CASE(7)
     HAMD_CASE(8) HAMD_CASE(9) HAMD_CASE(10) HAMD_CASE(11) HAMD_CASE(12) HAMD_CASE(13)
     HAMD_CASE(14) HAMD_CASE(15) HAMD_CASE(16)
@@ -478,30 +690,40 @@ int launch_hilb(const FormDev& d, const Slots& S, const StageArgs& A, hipStream_
   }
 }
 
-template <typename R>
-int launch_form(const FormDev& d, const Slots& S, const StageArgs& A, hipStream_t st) {
-  if (d.kind == 0) return launch_hilb<R>(d, S, A, st);
+template <typename R, bool F64>
+int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  if (d.kind == 0) return launch_hilb<R, F64>(d, A, st);
   if (d.kind == 1) {
     const int APB = 256 / d.N2;
-    int groups = (d.n_hrchy + APB - 1) / APB;
-    int grid = groups < kMaxGrid ? (groups < 1 ? 1 : groups) : kMaxGrid;
-    hipLaunchKernelGGL((liou_kernel<R>), dim3(grid), dim3(256), 0, st, d, S, A);
+    const int grid = cap_grid((d.n_hrchy + APB - 1) / APB);
+    hipLaunchKernelGGL((liou_kernel<R, F64>), dim3(grid), dim3(256), 0, st, d, A);
     return grid;
   }
-  long long blocks = (d.main_size + 255) / 256;
-  int grid = blocks < kMaxGrid ? static_cast<int>(blocks < 1 ? 1 : blocks) : kMaxGrid;
-  hipLaunchKernelGGL((ado_kernel<R>), dim3(grid), dim3(256), 0, st, d, S, A);
+  const int grid = cap_grid((d.main_size + 255) / 256);
+  hipLaunchKernelGGL((ado_kernel<R, F64>), dim3(grid), dim3(256), 0, st, d, A);
   return grid;
 }
 
 }  // namespace
 
-int launch_operator(const FormDev& d, int real_bytes, const Slots& S, const StageArgs& A,
-                    hipStream_t stream) {
-  int g = real_bytes == 4 ? launch_form<float>(d, S, A, stream)
-                          : launch_form<double>(d, S, A, stream);
+int launch_operator(const FormDev& d, int real_bytes, const StageArgs& A, hipStream_t stream) {
+  // every operand complex128: the epilogue needs no per-operand type checks
+  bool all64 = real_bytes == 8 && !A.in_lo && A.pre_lo == 0 && !A.k_lo;
+  for (int c = 0; c < 2; ++c) all64 = all64 && !A.dst_lo[c] && !A.dst2_lo[c];
+  int g;
+  if (real_bytes == 4)
+    g = launch_form<float, false>(d, A, stream);
+  else if (all64)
+    g = launch_form<double, true>(d, A, stream);
+  else
+    g = launch_form<double, false>(d, A, stream);
   HAMD_HIP(hipGetLastError());
   return g;
+}
+
+void launch_err_finalize(const StageArgs& A, int nparts, hipStream_t stream) {
+  hipLaunchKernelGGL(err_finalize_kernel, dim3(1), dim3(1024), 0, stream, A, nparts);
+  HAMD_HIP(hipGetLastError());
 }
 
 void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t stream) {
@@ -510,16 +732,13 @@ void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t
 }
 
 void launch_narrow(void* dst, const void* src, long long n, hipStream_t stream) {
-  long long b = (n + 255) / 256;
-  int grid = b < 4096 ? static_cast<int>(b < 1 ? 1 : b) : 4096;
-  hipLaunchKernelGGL(narrow_kernel, dim3(grid), dim3(256), 0, stream, static_cast<z64*>(dst),
-                     static_cast<const z128*>(src), n);
+  hipLaunchKernelGGL(narrow_kernel, dim3(cap_grid((n + 255) / 256)), dim3(256), 0, stream,
+                     static_cast<z64*>(dst), static_cast<const z128*>(src), n);
   HAMD_HIP(hipGetLastError());
 }
 
 void launch_copy(void* dst, const void* src, long long n, int type, hipStream_t stream) {
-  long long b = (n + 255) / 256;
-  int grid = b < 4096 ? static_cast<int>(b < 1 ? 1 : b) : 4096;
+  const int grid = cap_grid((n + 255) / 256);
   if (type == ST_C128)
     hipLaunchKernelGGL((copy_kernel<z128>), dim3(grid), dim3(256), 0, stream,
                        static_cast<z128*>(dst), static_cast<const z128*>(src), n);

@@ -265,6 +265,14 @@ void Qme::set_param() {
           break;
         }
 
+  real_v = diag_v;
+  for (int u = 0; u < nn && real_v; ++u)
+    for (int i = 0; i < N; ++i)
+      if (nz[u].V[i * N + i].im != 0.0) {
+        real_v = false;
+        break;
+      }
+
   noise_work_ = nz;
   H_work_ = Hw;
   build_links();
@@ -494,6 +502,18 @@ void Qme::upload() {
   HAMD_HIP(hipSetDevice(opt.device));
   d_link_ptr.upload(link_ptr);
   d_links.upload(links);
+  std::vector<LinkFlat> flat(links.size());
+  for (size_t l = 0; l < links.size(); ++l) {
+    const Coef& cf = coef[links[l].cid];
+    LinkFlat f{};
+    f.nbr = links[l].nbr;
+    f.u = cf.u;
+    f.iphi = z128{-cf.phi.im, cf.phi.re};  // i*phi
+    f.psi = cf.psi;
+    f.dir = cf.dir;
+    flat[l] = f;
+  }
+  d_flat.upload(flat);
   d_coef.upload(coef);
   d_gam.upload(ngamma);
   d_H.upload(H_work_);
@@ -548,6 +568,11 @@ void Qme::upload() {
   dev.dterm = d_dterm.as<z128>();
   dev.sdelta = d_sdelta.as<z128>();
   dev.diag_v = diag_v ? 1 : 0;
+  dev.real_v = real_v ? 1 : 0;
+  dev.flat = d_flat.as<LinkFlat>();
+  dev.max_links = 0;
+  for (int l = 0; l < hs.n_hrchy; ++l)
+    dev.max_links = std::max(dev.max_links, link_ptr[l + 1] - link_ptr[l]);
   dev.sop_ptr = d_sop_ptr.as<int>();
   dev.sop_col = d_sop_col.as<int>();
   dev.sop_val = d_sop_val.as<z128>();

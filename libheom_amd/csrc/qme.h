@@ -69,6 +69,7 @@ class Qme {
   std::vector<z128> ngamma;               // Gamma_n
   std::vector<std::vector<z128>> s, a;    // per noise, per mode
   bool diag_v = false;
+  bool real_v = false;  // diagonal V with real entries
 
   // neighbour links
   std::vector<int> link_ptr;
@@ -78,7 +79,7 @@ class Qme {
   // device copies
   FormDev dev{};
   hipStream_t stream = nullptr;
-  DevBuf d_link_ptr, d_links, d_coef, d_gam, d_H, d_V, d_Vd, d_dterm, d_sdelta;
+  DevBuf d_link_ptr, d_links, d_flat, d_coef, d_gam, d_H, d_V, d_Vd, d_dterm, d_sdelta;
   DevBuf d_sop_ptr, d_sop_col, d_sop_val;
   DevBuf d_R_ptr, d_R_col, d_R_val;
   long long ado_nnz = 0;

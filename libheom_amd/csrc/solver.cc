@@ -32,24 +32,32 @@ enum {
   SL_Y = 0, SL_YNEW = 1, SL_Y32 = 2, SL_YNEW32 = 3,
   SL_SA = 4, SL_SB = 5, SL_ACC = 6, SL_K1 = 7  // K1..K6 = 7..12
 };
-
-StageArgs base_args() {
-  StageArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.mode = MODE_STAGE;
-  a.k_slot = -1;
-  a.out_slot = -1;
-  a.combo[0].dst2 = -1;
-  a.combo[1].dst2 = -1;
-  return a;
-}
-
-void add_src(Combo& c, int slot, double coef) {
-  c.src[c.nsrc] = slot;
-  c.c[c.nsrc] = coef;
-  ++c.nsrc;
-}
 }  // namespace
+
+int StageSpec::pre_index(int slot) {
+  for (size_t s = 0; s < pre.size(); ++s)
+    if (pre[s] == slot) return static_cast<int>(s);
+  if (static_cast<int>(pre.size()) >= kMaxPre)
+    throw HeomError{HEOM_ERR_STATE, "stage needs too many operands"};
+  pre.push_back(slot);
+  return static_cast<int>(pre.size()) - 1;
+}
+
+// dst (dst2) <- sum of terms (in the given order) + ckv*k
+void StageSpec::combo(const std::vector<std::pair<int, double>>& terms, double ckv, int d,
+                      int d2) {
+  const int c = ncombo++;
+  int last = -1;
+  for (auto& t : terms) {
+    const int s = pre_index(t.first);
+    if (s <= last) throw HeomError{HEOM_ERR_STATE, "combination order differs from operands"};
+    last = s;
+    cc[c][s] = t.second;
+  }
+  ck[c] = ckv;
+  dst[c] = d;
+  dst2[c] = d2;
+}
 
 Solver::Solver(Qme* q_, int kind_) : q(q_), kind(kind_) {
   if (!q->ready) throw HeomError{HEOM_ERR_STATE, "solver: set_param has not been called"};
@@ -58,36 +66,31 @@ Solver::Solver(Qme* q_, int kind_) : q(q_), kind(kind_) {
   HAMD_HIP(hipSetDevice(q->opt.device));
   in_bytes = q->dtype == HEOM_C64 ? 4 : 8;
   ms = q->main_size();
-  const size_t z128b = sizeof(z128) * ms, sinb = (in_bytes == 4 ? sizeof(z64) : sizeof(z128)) * ms;
-  const int tin = in_bytes == 4 ? ST_C64 : ST_C128;
-  for (int s = 0; s < kMaxSlots; ++s) {
-    slots.p[s] = nullptr;
-    slots.type[s] = ST_C128;
-  }
-  auto mk = [&](int s, size_t b, int t) {
+  const size_t z128b = sizeof(z128) * ms;
+  const size_t sinb = (in_bytes == 4 ? sizeof(z64) : sizeof(z128)) * ms;
+  const int lo = in_bytes == 4 ? 1 : 0;
+  auto mk = [&](int s, size_t b, int is_lo) {
     P[s].alloc(b);
     HAMD_HIP(hipMemset(P[s].p, 0, b));
-    slots.p[s] = P[s].p;
-    slots.type[s] = t;
+    sp[s] = P[s].p;
+    slo[s] = is_lo;
   };
-  mk(SL_Y, z128b, ST_C128);
-  if (kind == HEOM_RKDP) mk(SL_YNEW, z128b, ST_C128);
-  if (in_bytes == 4) {
-    mk(SL_Y32, sinb, tin);
-    if (kind == HEOM_RKDP) mk(SL_YNEW32, sinb, tin);
+  mk(SL_Y, z128b, 0);
+  if (kind == HEOM_RKDP) mk(SL_YNEW, z128b, 0);
+  if (lo) {
+    mk(SL_Y32, sinb, 1);
+    if (kind == HEOM_RKDP) mk(SL_YNEW32, sinb, 1);
   }
-  mk(SL_SA, sinb, tin);
-  mk(SL_SB, sinb, tin);
+  mk(SL_SA, sinb, lo);
+  mk(SL_SB, sinb, lo);
   if (kind == HEOM_RK4) {
-    mk(SL_ACC, z128b, ST_C128);
+    mk(SL_ACC, z128b, 0);
   } else {
-    for (int k = 0; k < 6; ++k) mk(SL_K1 + k, sinb, tin);
+    for (int k = 0; k < 6; ++k) mk(SL_K1 + k, sinb, lo);
   }
   d_ctrl.alloc(sizeof(Ctrl));
   HAMD_HIP(hipMemset(d_ctrl.p, 0, sizeof(Ctrl)));
   d_partials.alloc(sizeof(double) * kMaxGrid);
-  d_counter.alloc(sizeof(unsigned) * 4);
-  HAMD_HIP(hipMemset(d_counter.p, 0, sizeof(unsigned) * 4));
   HAMD_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ctrl), sizeof(Ctrl), hipHostMallocDefault));
   std::memset(h_ctrl, 0, sizeof(Ctrl));
   build_stages(0.0, 0.0);
@@ -97,110 +100,133 @@ Solver::~Solver() {
   if (h_ctrl) (void)hipHostFree(h_ctrl);
 }
 
+// Physical buffer of a slot for ping-pong parity `cur` (slots 0..3 swap
+// within their pair when cur == 1).
+void* Solver::slot_ptr(int s, int cur) const {
+  if (s < 0) return nullptr;
+  if (s < 4) return sp[(s & 2) + ((s & 1) ^ cur)];
+  return sp[s];
+}
+
+StageArgs Solver::resolve(const StageSpec& s) const {
+  StageArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.mode = MODE_STAGE;
+  const bool pp = kind == HEOM_RKDP;  // ping-pong pairs only with the control block
+  for (int cur = 0; cur < 2; ++cur) {
+    const int c = pp ? cur : 0;
+    a.in[cur] = slot_ptr(s.in_slot, c);
+    a.k[cur] = slot_ptr(s.k_slot, c);
+    for (size_t j = 0; j < s.pre.size(); ++j) a.pre[cur][j] = slot_ptr(s.pre[j], c);
+    for (int k = 0; k < 2; ++k) {
+      a.dst[k][cur] = slot_ptr(s.dst[k], c);
+      a.dst2[k][cur] = slot_ptr(s.dst2[k], c);
+    }
+  }
+  a.in_lo = slo[s.in_slot];
+  a.npre = static_cast<int>(s.pre.size());
+  for (size_t j = 0; j < s.pre.size(); ++j)
+    if (s.pre[j] >= 0 && slo[s.pre[j]]) a.pre_lo |= 1u << j;
+  a.k_lo = s.k_slot >= 0 ? slo[s.k_slot] : 0;
+  a.ncombo = s.ncombo;
+  std::memcpy(a.cc, s.cc, sizeof(a.cc));
+  std::memcpy(a.ck, s.ck, sizeof(a.ck));
+  for (int k = 0; k < 2; ++k) {
+    a.dst_lo[k] = s.dst[k] >= 0 ? slo[s.dst[k]] : 0;
+    a.dst2_lo[k] = s.dst2[k] >= 0 ? slo[s.dst2[k]] : 0;
+  }
+  a.do_err = s.do_err;
+  std::memcpy(a.ec, s.ec, sizeof(a.ec));
+  a.ek = s.ek;
+  a.ynew_pre = s.ynew_pre;
+  a.yold_pre = s.yold_pre;
+  a.control = s.control;
+  a.inv_n = 1.0 / static_cast<double>(ms);
+  if (kind == HEOM_RKDP) a.ctrl = d_ctrl.as<Ctrl>();
+  a.partials = d_partials.as<double>();
+  a.max_partials = kMaxGrid;
+  return a;
+}
+
 void Solver::build_stages(double atol, double rtol) {
+  specs.clear();
   stages.clear();
   const bool lo = in_bytes == 4;
   const int y_in = lo ? SL_Y32 : SL_Y;
   if (kind == HEOM_RK4) {
-    // rk4.h:47-64 with k2/k3 folded into the accumulator as they appear
-    StageArgs a = base_args();
+    // rk4.h:47-64; k2 and k3 enter the accumulator as soon as they exist
+    StageSpec a;
     a.in_slot = y_in;
-    a.ncombo = 2;
-    add_src(a.combo[0], SL_Y, 1.0);
-    a.combo[0].ck = 0.5;
-    a.combo[0].dst = SL_SA;
-    add_src(a.combo[1], SL_Y, 1.0);
-    a.combo[1].ck = 1.0 / 6.0;
-    a.combo[1].dst = SL_ACC;
-    stages.push_back(a);
+    a.combo({{SL_Y, 1.0}}, 0.5, SL_SA);
+    a.combo({{SL_Y, 1.0}}, 1.0 / 6.0, SL_ACC);
+    specs.push_back(a);
 
-    a = base_args();
+    a = StageSpec();
     a.in_slot = SL_SA;
-    a.ncombo = 2;
-    add_src(a.combo[0], SL_Y, 1.0);
-    a.combo[0].ck = 0.5;
-    a.combo[0].dst = SL_SB;
-    add_src(a.combo[1], SL_ACC, 1.0);
-    a.combo[1].ck = 2.0 / 6.0;
-    a.combo[1].dst = SL_ACC;
-    stages.push_back(a);
+    a.combo({{SL_Y, 1.0}}, 0.5, SL_SB);
+    a.combo({{SL_ACC, 1.0}}, 2.0 / 6.0, SL_ACC);
+    specs.push_back(a);
 
-    a = base_args();
+    a = StageSpec();
     a.in_slot = SL_SB;
-    a.ncombo = 2;
-    add_src(a.combo[0], SL_Y, 1.0);
-    a.combo[0].ck = 1.0;
-    a.combo[0].dst = SL_SA;
-    add_src(a.combo[1], SL_ACC, 1.0);
-    a.combo[1].ck = 2.0 / 6.0;
-    a.combo[1].dst = SL_ACC;
-    stages.push_back(a);
+    a.combo({{SL_Y, 1.0}}, 1.0, SL_SA);
+    a.combo({{SL_ACC, 1.0}}, 2.0 / 6.0, SL_ACC);
+    specs.push_back(a);
 
-    a = base_args();
+    a = StageSpec();
     a.in_slot = SL_SA;
-    a.ncombo = 1;
-    add_src(a.combo[0], SL_ACC, 1.0);
-    a.combo[0].ck = 1.0 / 6.0;
-    a.combo[0].dst = SL_Y;
-    a.combo[0].dst2 = lo ? SL_Y32 : -1;
-    stages.push_back(a);
-    return;
-  }
-  // Dormand-Prince: stage i (0-based) evaluates k_i at its input and writes
-  // the input of stage i+1 (rkdp.h:111-127); stage 5 also forms the new state
-  // (rkdp.h:130-132); stage 6 evaluates k_7 at the new state and the error
-  // norm (rkdp.h:136-153).
-  const int sin_[6] = {y_in, SL_SA, SL_SB, SL_SA, SL_SB, SL_SA};
-  const int sout[5] = {SL_SA, SL_SB, SL_SA, SL_SB, SL_SA};
-  for (int i = 0; i < 6; ++i) {
-    StageArgs a = base_args();
-    a.in_slot = sin_[i];
-    a.k_slot = SL_K1 + i;
-    a.ncombo = 1;
-    Combo& c = a.combo[0];
-    add_src(c, SL_Y, 1.0);
-    if (i < 5) {
-      for (int j = 0; j < i; ++j) add_src(c, SL_K1 + j, DP_A[i + 1][j]);
-      c.ck = DP_A[i + 1][i];
-      c.dst = sout[i];
-    } else {
-      for (int j = 0; j < 5; ++j)
-        if (DP_B[j] != 0.0) add_src(c, SL_K1 + j, DP_B[j]);
-      c.ck = DP_B[5];
-      c.dst = SL_YNEW;
-      c.dst2 = lo ? SL_YNEW32 : -1;
+    a.combo({{SL_ACC, 1.0}}, 1.0 / 6.0, SL_Y, lo ? SL_Y32 : -1);
+    specs.push_back(a);
+  } else {
+    // Dormand-Prince: stage i (0-based) evaluates k_i at its input and writes
+    // the input of stage i+1 (rkdp.h:111-127); stage 5 also forms the new
+    // state (rkdp.h:130-132); stage 6 evaluates k_7 at the new state and the
+    // error norm (rkdp.h:136-153).
+    const int sin_[6] = {y_in, SL_SA, SL_SB, SL_SA, SL_SB, SL_SA};
+    const int sout[5] = {SL_SA, SL_SB, SL_SA, SL_SB, SL_SA};
+    for (int i = 0; i < 6; ++i) {
+      StageSpec a;
+      a.in_slot = sin_[i];
+      a.k_slot = SL_K1 + i;
+      std::vector<std::pair<int, double>> terms{{SL_Y, 1.0}};
+      if (i < 5) {
+        for (int j = 0; j < i; ++j) terms.push_back({SL_K1 + j, DP_A[i + 1][j]});
+        a.combo(terms, DP_A[i + 1][i], sout[i]);
+      } else {
+        for (int j = 0; j < 5; ++j)
+          if (DP_B[j] != 0.0) terms.push_back({SL_K1 + j, DP_B[j]});
+        a.combo(terms, DP_B[5], SL_YNEW, lo ? SL_YNEW32 : -1);
+      }
+      specs.push_back(a);
     }
-    stages.push_back(a);
+    StageSpec a;
+    a.in_slot = lo ? SL_YNEW32 : SL_YNEW;
+    a.do_err = 1;
+    for (int j = 0; j < 6; ++j)
+      if (DP_E[j] != 0.0) a.ec[a.pre_index(SL_K1 + j)] = DP_E[j];
+    a.ek = DP_E[6];
+    // the error weights read y_new / y_old from the last two operand positions
+    while (static_cast<int>(a.pre.size()) < kMaxPre - 2) a.pre.push_back(-1);
+    a.pre.push_back(SL_YNEW);
+    a.pre.push_back(SL_Y);
+    a.ynew_pre = kMaxPre - 2;
+    a.yold_pre = kMaxPre - 1;
+    a.control = 1;
+    specs.push_back(a);
   }
-  StageArgs a = base_args();
-  a.in_slot = lo ? SL_YNEW32 : SL_YNEW;
-  a.do_err = 1;
-  for (int j = 0; j < 6; ++j)
-    if (DP_E[j] != 0.0) {
-      a.esrc[a.ne] = SL_K1 + j;
-      a.ec[a.ne] = DP_E[j];
-      ++a.ne;
-    }
-  a.ek = DP_E[6];
-  a.ynew_slot = SL_YNEW;
-  a.yold_slot = SL_Y;
-  a.atol = atol;
-  a.rtol = rtol;
-  a.control = 1;
-  a.inv_n = 1.0 / static_cast<double>(ms);
-  stages.push_back(a);
-  for (auto& s : stages) {
-    s.ctrl = d_ctrl.as<Ctrl>();
-    s.partials = d_partials.as<double>();
-    s.counter = d_counter.as<unsigned>();
-    s.max_partials = kMaxGrid;
+  for (auto& s : specs) {
+    StageArgs r = resolve(s);
+    r.atol = atol;
+    r.rtol = rtol;
+    stages.push_back(r);
   }
 }
 
 void Solver::launch_stage(int s, double dt) {
   StageArgs a = stages[s];
   a.a_re = dt;
-  launch_operator(q->dev, in_bytes, slots, a, q->stream);
+  int grid = launch_operator(q->dev, in_bytes, a, q->stream);
+  if (a.do_err) launch_err_finalize(a, grid, q->stream);
 }
 
 // fixed_step_size_solver::solve_1 (fixed_step_size_solver.h:29-52): steps of
@@ -301,7 +327,7 @@ void Solver::time_stages(int reps, std::vector<double>& out) {
   const int ns = n_stages();
   out.assign(ns, 0.0);
   if (kind == HEOM_RKDP) {
-    // a control block that never finishes; stage 7 only reduces the norm
+    // a control block that never finishes; the last stage only reduces the norm
     Ctrl c{};
     c.t = 0.0;
     c.t_bound = 1e300;
@@ -317,9 +343,9 @@ void Solver::time_stages(int reps, std::vector<double>& out) {
     StageArgs a = stages[s];
     a.control = 0;
     a.a_re = 1e-3;
-    launch_operator(q->dev, in_bytes, slots, a, st);  // warm-up
+    launch_operator(q->dev, in_bytes, a, st);  // warm-up
     HAMD_HIP(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) launch_operator(q->dev, in_bytes, slots, a, st);
+    for (int r = 0; r < reps; ++r) launch_operator(q->dev, in_bytes, a, st);
     HAMD_HIP(hipEventRecord(e1, st));
     HAMD_HIP(hipEventSynchronize(e1));
     float msf = 0.f;
@@ -332,32 +358,33 @@ void Solver::time_stages(int reps, std::vector<double>& out) {
 
 // Algorithmic HBM bytes of each stage pass: every state-sized buffer the
 // pass reads or writes counts once per element (neighbour reads of the stage
-// input are another ADO's compulsory read), plus the per-ADO link tables.
+// input are another ADO's compulsory read), plus the operator tables.
 void Solver::stage_bytes(std::vector<double>& out) {
   const int ns = n_stages();
   out.assign(ns, 0.0);
-  auto size_of = [&](int slot) { return slots.type[slot] == ST_C128 ? 16.0 : 8.0; };
-  const double tables = 8.0 * q->links.size() + 4.0 * (q->hs.n_hrchy + 1) +
-                        16.0 * q->hs.n_hrchy + sizeof(Coef) * q->coef.size();
+  auto size_of = [&](int slot) { return slo[slot] ? 8.0 : 16.0; };
+  double tables = 0.0;
+  if (q->kind == HEOM_ADO) {
+    tables = 20.0 * q->ado_nnz + 8.0 * (ms + 1);
+  } else {
+    const double lrec = (q->kind == HEOM_HILB && q->diag_v) ? sizeof(LinkFlat) : sizeof(Link);
+    tables = lrec * q->links.size() + 4.0 * (q->hs.n_hrchy + 1) + 16.0 * q->hs.n_hrchy;
+  }
   for (int s = 0; s < ns; ++s) {
-    const StageArgs& a = stages[s];
+    const StageSpec& a = specs[s];
     std::set<int> rd, wr;
     rd.insert(a.in_slot);
+    for (int j : a.pre)
+      if (j >= 0) rd.insert(j);
     if (a.k_slot >= 0) wr.insert(a.k_slot);
     for (int c = 0; c < a.ncombo; ++c) {
-      for (int j = 0; j < a.combo[c].nsrc; ++j) rd.insert(a.combo[c].src[j]);
-      wr.insert(a.combo[c].dst);
-      if (a.combo[c].dst2 >= 0) wr.insert(a.combo[c].dst2);
-    }
-    if (a.do_err) {
-      for (int j = 0; j < a.ne; ++j) rd.insert(a.esrc[j]);
-      rd.insert(a.ynew_slot);
-      rd.insert(a.yold_slot);
+      wr.insert(a.dst[c]);
+      if (a.dst2[c] >= 0) wr.insert(a.dst2[c]);
     }
     double per = 0.0;
     for (int r : rd) per += size_of(r);
     for (int w : wr) per += size_of(w);
-    out[s] = per * static_cast<double>(ms) + (q->kind == HEOM_ADO ? 0.0 : tables);
+    out[s] = per * static_cast<double>(ms) + tables;
   }
 }
 

@@ -8,6 +8,26 @@
 
 namespace hamd {
 
+// Slot-level description of one stage pass; resolved into a StageArgs with
+// device pointers for both ping-pong parities.
+struct StageSpec {
+  int in_slot = 0;
+  int k_slot = -1;
+  std::vector<int> pre;  // operand slots, in summation order
+  int ncombo = 0;
+  double cc[2][kMaxPre] = {};
+  double ck[2] = {};
+  int dst[2] = {-1, -1};
+  int dst2[2] = {-1, -1};
+  int do_err = 0;
+  double ec[kMaxPre] = {};
+  double ek = 0.0;
+  int ynew_pre = -1, yold_pre = -1;
+  int control = 0;
+  int pre_index(int slot);
+  void combo(const std::vector<std::pair<int, double>>& terms, double ckv, int d, int d2 = -1);
+};
+
 class Solver {
  public:
   Solver(Qme* q, int kind);
@@ -31,13 +51,20 @@ class Solver {
   int n_stages() const { return kind == HEOM_RK4 ? 4 : 7; }
 
  private:
-  DevBuf P[kMaxSlots];
-  Slots slots{};
-  DevBuf d_ctrl, d_partials, d_counter;
+  static constexpr int kSlots = 16;
+  // slot table: slots 0..3 are ping-pong pairs (see common.h), resolved per
+  // parity when a stage's pointers are built
+  DevBuf P[kSlots];
+  void* sp[kSlots] = {};
+  int slo[kSlots] = {};  // 1: complex64 buffer
+  DevBuf d_ctrl, d_partials;
   Ctrl* h_ctrl = nullptr;  // pinned mirror
   std::vector<StageArgs> stages;
 
   void build_stages(double atol, double rtol);
+  void* slot_ptr(int s, int cur) const;
+  StageArgs resolve(const StageSpec& s) const;
+  std::vector<StageSpec> specs;
   void launch_stage(int s, double dt);
   void rk4_interval(double t0, double t1, double dt);
   int rkdp_interval(double t0, double t1, double dt_first, int& batch);

@@ -159,6 +159,10 @@ struct FormDev {
   int real_v;            // diagonal V with real entries
   int max_links;         // most links of any single ADO
   const LinkFlat* flat;  // links with inlined coefficients (same order as links)
+  // diagonal V: per noise u (plus one zero row) and element e = i*N+j the pair
+  // (v_i - v_j, v_i + v_j); real V stores the pair as (re, im) of one z128,
+  // complex V as two z128
+  const z128* cmac_tab;
   // liou: CSR superoperators (N2 x N2): R0, then Phi_u, Psi_u
   const int* sop_ptr;    // (1 + 2*n_noise) * (N2 + 1)
   const int* sop_col;

@@ -67,15 +67,17 @@ __device__ __forceinline__ T pick(const T (&a)[2], int cur) {
 
 // Per-element epilogue operands, loaded before the operator work so that
 // their memory latency overlaps it.  F64: every operand is complex128.
-struct Pre {
-  z128 v[kMaxPre];
+template <int KP>
+struct PreT {
+  z128 v[KP];
 };
+using Pre = PreT<kMaxPre>;
 
-template <bool F64>
-__device__ __forceinline__ void prefetch(Pre& p, const StageArgs& A, int cur, long long gi,
+template <bool F64, int KP>
+__device__ __forceinline__ void prefetch(PreT<KP>& p, const StageArgs& A, int cur, long long gi,
                                          bool ok) {
 #pragma unroll
-  for (int s = 0; s < kMaxPre; ++s) {
+  for (int s = 0; s < KP; ++s) {
     const void* ptr = cur ? A.pre[1][s] : A.pre[0][s];
     if (ok && s < A.npre && ptr != nullptr)
       p.v[s] = ldz(ptr, F64 ? false : ((A.pre_lo >> s) & 1u), gi);
@@ -87,11 +89,12 @@ __device__ __forceinline__ void prefetch(Pre& p, const StageArgs& A, int cur, lo
 // Linear combination sum_s c[s]*p[s] + ck*k, accumulated in s order.  Terms
 // with a zero coefficient add an exact zero, so the result equals the
 // sequential sum over the non-zero terms (reference order).
-__device__ __forceinline__ z128 combine(const Pre& p, const double* c, int npre, double ck,
+template <int KP>
+__device__ __forceinline__ z128 combine(const PreT<KP>& p, const double* c, int npre, double ck,
                                         z128 k) {
   z128 v{0.0, 0.0};
 #pragma unroll
-  for (int s = 0; s < kMaxPre; ++s) {
+  for (int s = 0; s < KP; ++s) {
     if (s < npre) {
       v.re = fma(c[s], p.v[s].re, v.re);
       v.im = fma(c[s], p.v[s].im, v.im);
@@ -104,8 +107,8 @@ __device__ __forceinline__ z128 combine(const Pre& p, const double* c, int npre,
 
 // Epilogue shared by all operator kernels.  D is the operator applied to the
 // stage input at flat element gi; returns w^2 of the error norm (or 0).
-template <bool F64, typename R>
-__device__ __forceinline__ double epilogue(const Pre& p, const StageArgs& A, int cur,
+template <bool F64, typename R, int KP>
+__device__ __forceinline__ double epilogue(const PreT<KP>& p, const StageArgs& A, int cur,
                                            long long gi, cx<R> Dr, z128 alpha) {
   const z128 D = conv<double>(Dr);
   const z128 k{-(alpha.re * D.re - alpha.im * D.im), -(alpha.re * D.im + alpha.im * D.re)};
@@ -127,10 +130,10 @@ __device__ __forceinline__ double epilogue(const Pre& p, const StageArgs& A, int
       if (void* d2 = pick(A.dst2[c], cur)) stz(d2, F64 ? false : A.dst2_lo[c], gi, v);
     }
   }
-  if (A.do_err) {
-    // operands kMaxPre-2 / kMaxPre-1 hold y_new / y_old (zero weights in ec)
-    const z128 e = combine(p, A.ec, kMaxPre - 2, A.ek, k);
-    const double a1 = zabs(p.v[kMaxPre - 2]), a0 = zabs(p.v[kMaxPre - 1]);
+  if (KP == kMaxPre && A.do_err) {
+    // operands KP-2 / KP-1 hold y_new / y_old (zero weights in ec)
+    const z128 e = combine(p, A.ec, KP - 2, A.ek, k);
+    const double a1 = zabs(p.v[KP - 2]), a0 = zabs(p.v[KP - 1]);
     const double w = zabs(e) / (A.atol + A.rtol * (a1 > a0 ? a1 : a0));
     return w * w;
   }
@@ -252,40 +255,13 @@ struct WaveCfg {
   static constexpr int BS = ((N2 + 63) / 64) * 64;
 };
 
-// cm/ac table in LDS: per noise u and element e the pair (v_i - v_j, v_i + v_j)
-// (real V: 2 reals; complex V: 2 complex), plus one zero row for direct links.
-template <typename R, bool REALV>
-size_t fast_lds_bytes(int n_noise, int N2) {
-  return sizeof(R) * (REALV ? 2 : 4) * static_cast<size_t>(n_noise + 1) * N2;
-}
-
-// coefficient of one link at element `base` = u*N2 + e of the cm/ac table
-template <typename R, bool REALV>
-__device__ __forceinline__ cx<R> fast_coef(const R* s_cm, const LinkFlat& q, int base) {
-  cx<R> cf;
-  if (REALV) {
-    const R cm = s_cm[2 * base], ac = s_cm[2 * base + 1];
-    cf.re = cm * static_cast<R>(q.iphi.re) + ac * static_cast<R>(q.psi.re) +
-            static_cast<R>(q.dir.re);
-    cf.im = cm * static_cast<R>(q.iphi.im) + ac * static_cast<R>(q.psi.im) +
-            static_cast<R>(q.dir.im);
-  } else {
-    const cx<R> cm{s_cm[4 * base], s_cm[4 * base + 1]};
-    const cx<R> ac{s_cm[4 * base + 2], s_cm[4 * base + 3]};
-    cf = cm * conv<R>(q.iphi) + ac * conv<R>(q.psi) + conv<R>(q.dir);
-  }
-  return cf;
-}
-
-template <int N, typename R, bool REALV, bool F64>
+template <int N, typename R, bool REALV, bool F64, int KP>
 __global__ void __launch_bounds__(WaveCfg<N>::BS)
     hilb_fast_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   constexpr int N2 = N * N, BS = WaveCfg<N>::BS;
   __shared__ C s_rho[N2];
   __shared__ C s_H[N2];
-  extern __shared__ __align__(16) unsigned char s_dyn[];
-  R* s_cm = reinterpret_cast<R*>(s_dyn);
 
   const Prologue pg = prologue(A);
   if (pg.skip) return;
@@ -294,32 +270,6 @@ __global__ void __launch_bounds__(WaveCfg<N>::BS)
   const bool lane_ok = e < N2;
   const int ee = lane_ok ? e : 0;
   const int i = ee / N, j = ee - (ee / N) * N;
-  const int nn = d.n_noise;
-  for (int x = e; x < N2; x += BS) s_H[x] = conv<R>(d.H[x]);
-  for (int x = e; x < (nn + 1) * N2; x += BS) {
-    const int u = x / N2, y = x - u * N2;
-    const int ii = y / N, jj = y - (y / N) * N;
-    if (REALV) {
-      R cm = 0, ac = 0;
-      if (u < nn) {
-        cm = static_cast<R>(d.Vd[u * N + ii].re - d.Vd[u * N + jj].re);
-        ac = static_cast<R>(d.Vd[u * N + ii].re + d.Vd[u * N + jj].re);
-      }
-      s_cm[2 * x] = cm;
-      s_cm[2 * x + 1] = ac;
-    } else {
-      C cm{0, 0}, ac{0, 0};
-      if (u < nn) {
-        cm = conv<R>(d.Vd[u * N + ii] - d.Vd[u * N + jj]);
-        ac = conv<R>(d.Vd[u * N + ii] + d.Vd[u * N + jj]);
-      }
-      s_cm[4 * x] = cm.re;
-      s_cm[4 * x + 1] = cm.im;
-      s_cm[4 * x + 2] = ac.re;
-      s_cm[4 * x + 3] = ac.im;
-    }
-  }
-  const C dterm = lane_ok ? conv<R>(d.dterm[ee]) : C{0, 0};
   const C* in = static_cast<const C*>(pick(A.in, cur));
   const auto link_ptr = cst(d.link_ptr);
   const auto flat = cst(d.flat);
@@ -329,12 +279,15 @@ __global__ void __launch_bounds__(WaveCfg<N>::BS)
   const int g = blockIdx.x;
   const long long gi = static_cast<long long>(g) * N2 + ee;
   const C r = in[gi];
-  Pre pv;
-  prefetch<F64>(pv, A, cur, gi, true);
+  PreT<KP> pv;
+  prefetch<F64, KP>(pv, A, cur, gi, true);
   const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
   const C gm = conv<R>(z128{gam[g].re, gam[g].im});
+  const C dterm = conv<R>(d.dterm[ee]);
+  for (int x2 = e; x2 < N2; x2 += BS) s_H[x2] = conv<R>(d.H[x2]);
   if (lane_ok) s_rho[e] = r;
   __syncthreads();
+
   C acc{0, 0};
 #pragma unroll
   for (int k = 0; k < N; ++k) {
@@ -344,24 +297,48 @@ __global__ void __launch_bounds__(WaveCfg<N>::BS)
   }
   C D = times_i(acc);
   cmac(D, gm + dterm, r);
-  int l = l0;
-  for (; l + 4 <= l1; l += 4) {
+  const z128* cmt = d.cmac_tab;  // (n_noise+1) x N2 pairs (v_i - v_j, v_i + v_j)
+  for (int l = l0; l < l1; l += 4) {
+    const int nb = (l1 - l < 4) ? l1 - l : 4;
     LinkFlat q[4];
     C x[4];
+    z128 cm[4], ac[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) q[t] = ld_flat(flat + l + t);
+    for (int t = 0; t < 4; ++t) {
+      if (t < nb) {
+        q[t] = ld_flat(flat + l + t);
+        x[t] = in[static_cast<long long>(q[t].nbr) * N2 + ee];
+        const long long b = static_cast<long long>(q[t].u) * N2 + ee;
+        if (REALV) {
+          const z128 p = cmt[b];  // (cm, ac) as two reals
+          cm[t] = z128{p.re, 0.0};
+          ac[t] = z128{p.im, 0.0};
+        } else {
+          cm[t] = cmt[2 * b];
+          ac[t] = cmt[2 * b + 1];
+        }
+      }
+    }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) x[t] = in[static_cast<long long>(q[t].nbr) * N2 + ee];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) cmac(D, fast_coef<R, REALV>(s_cm, q[t], q[t].u * N2 + ee), x[t]);
-  }
-  for (; l < l1; ++l) {
-    const LinkFlat q = ld_flat(flat + l);
-    const C x = in[static_cast<long long>(q.nbr) * N2 + ee];
-    cmac(D, fast_coef<R, REALV>(s_cm, q, q.u * N2 + ee), x);
+    for (int t = 0; t < 4; ++t) {
+      if (t < nb) {
+        C cf;
+        if (REALV) {
+          const R c0 = static_cast<R>(cm[t].re), a0 = static_cast<R>(ac[t].re);
+          cf.re = c0 * static_cast<R>(q[t].iphi.re) + a0 * static_cast<R>(q[t].psi.re) +
+                  static_cast<R>(q[t].dir.re);
+          cf.im = c0 * static_cast<R>(q[t].iphi.im) + a0 * static_cast<R>(q[t].psi.im) +
+                  static_cast<R>(q[t].dir.im);
+        } else {
+          cf = conv<R>(cm[t]) * conv<R>(q[t].iphi) + conv<R>(ac[t]) * conv<R>(q[t].psi) +
+               conv<R>(q[t].dir);
+        }
+        cmac(D, cf, x[t]);
+      }
+    }
   }
   double w2 = 0.0;
-  if (lane_ok) w2 = epilogue<F64, R>(pv, A, cur, gi, D, pg.alpha);
+  if (lane_ok) w2 = epilogue<F64, R, KP>(pv, A, cur, gi, D, pg.alpha);
   if (A.do_err) write_partial(A, w2);
 }
 
@@ -413,7 +390,7 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
     const long long gi = static_cast<long long>(n) * N2 + e;
     const C r = ok ? in[gi] : C{0, 0};
     Pre pv;
-    prefetch<F64>(pv, A, cur, gi, ok);
+    prefetch<F64, kMaxPre>(pv, A, cur, gi, ok);
     const int l0 = ok ? d.link_ptr[n] : 0;
     const int l1 = ok ? d.link_ptr[n + 1] : 0;
     if (lane_ok) s_rho[tid] = r;
@@ -503,7 +480,7 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
         cmac(D, conv<R>(c.dir), in[static_cast<long long>(lk.nbr) * N2 + e]);
       }
     }
-    if (ok) err_acc += epilogue<F64, R>(pv, A, cur, gi, D, pg.alpha);
+    if (ok) err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, gi, D, pg.alpha);
     __syncthreads();
   }
   if (A.do_err) write_partial(A, err_acc);
@@ -537,7 +514,7 @@ __global__ void __launch_bounds__(256) liou_kernel(const FormDev d, const StageA
     const long long gi = static_cast<long long>(n) * N2 + e;
     const C r = ok ? in[gi] : C{0, 0};
     Pre pv;
-    prefetch<F64>(pv, A, cur, gi, ok);
+    prefetch<F64, kMaxPre>(pv, A, cur, gi, ok);
     if (lane_ok) s_rho[tid] = r;
     __syncthreads();
     C D{0, 0};
@@ -581,7 +558,7 @@ __global__ void __launch_bounds__(256) liou_kernel(const FormDev d, const StageA
       const Coef c = d.coef[lk.cid];
       cmac(D, conv<R>(c.dir), in[static_cast<long long>(lk.nbr) * N2 + e]);
     }
-    if (ok) err_acc += epilogue<F64, R>(pv, A, cur, gi, D, pg.alpha);
+    if (ok) err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, gi, D, pg.alpha);
     __syncthreads();
   }
   if (A.do_err) write_partial(A, err_acc);
@@ -602,11 +579,11 @@ __global__ void __launch_bounds__(256) ado_kernel(const FormDev d, const StageAr
   for (long long r = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; r < rows;
        r += static_cast<long long>(gridDim.x) * blockDim.x) {
     Pre pv;
-    prefetch<F64>(pv, A, cur, r, true);
+    prefetch<F64, kMaxPre>(pv, A, cur, r, true);
     C D{0, 0};
     for (long long p = d.R_ptr[r]; p < d.R_ptr[r + 1]; ++p)
       cmac(D, conv<R>(d.R_val[p]), in[d.R_col[p]]);
-    err_acc += epilogue<F64, R>(pv, A, cur, r, D, pg.alpha);
+    err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, r, D, pg.alpha);
   }
   if (A.do_err) write_partial(A, err_acc);
 }
@@ -646,20 +623,24 @@ int launch_hilb_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
   if (N >= 6 && d.diag_v && d.n_hrchy <= kMaxGrid) {
     constexpr int BS = WaveCfg<N>::BS;
     const int grid = d.n_hrchy;
+    // four prefetched operands cover every stage except the last Dormand-Prince one
+    const bool small = A.npre <= 4 && !A.do_err;
     if (d.real_v) {
-      const size_t lds = fast_lds_bytes<R, true>(d.n_noise, N * N);
-      if (lds <= 48 * 1024) {
-        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64>), dim3(grid), dim3(BS), lds, st, d,
-                           A);
-        return grid;
-      }
-    } else {
-      const size_t lds = fast_lds_bytes<R, false>(d.n_noise, N * N);
-      if (lds <= 48 * 1024) {
-        hipLaunchKernelGGL((hilb_fast_kernel<N, R, false, F64>), dim3(grid), dim3(BS), lds, st,
+      if (small)
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, 4>), dim3(grid), dim3(BS), 0, st,
                            d, A);
-        return grid;
-      }
+      else
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, kMaxPre>), dim3(grid), dim3(BS), 0,
+                           st, d, A);
+      return grid;
+    } else {
+      if (small)
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, false, F64, 4>), dim3(grid), dim3(BS), 0, st,
+                           d, A);
+      else
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, false, F64, kMaxPre>), dim3(grid), dim3(BS), 0,
+                           st, d, A);
+      return grid;
     }
   }
   constexpr int APB = HilbCfg<N>::APB, BS = HilbCfg<N>::BS;

@@ -514,6 +514,27 @@ void Qme::upload() {
     flat[l] = f;
   }
   d_flat.upload(flat);
+  {
+    // (v_i - v_j, v_i + v_j) per noise and element, one zero row for direct links
+    std::vector<z128> tab;
+    if (diag_v) {
+      const size_t rows = static_cast<size_t>(nn + 1) * N2;
+      tab.assign(real_v ? rows : 2 * rows, z128{0.0, 0.0});
+      for (int u = 0; u < nn; ++u)
+        for (int i = 0; i < N; ++i)
+          for (int j = 0; j < N; ++j) {
+            const z128 vi = noise_work_[u].V[i * N + i], vj = noise_work_[u].V[j * N + j];
+            const size_t b = static_cast<size_t>(u) * N2 + i * N + j;
+            if (real_v) {
+              tab[b] = z128{vi.re - vj.re, vi.re + vj.re};
+            } else {
+              tab[2 * b] = vi - vj;
+              tab[2 * b + 1] = vi + vj;
+            }
+          }
+    }
+    d_cmac.upload(tab);
+  }
   d_coef.upload(coef);
   d_gam.upload(ngamma);
   d_H.upload(H_work_);
@@ -570,6 +591,7 @@ void Qme::upload() {
   dev.diag_v = diag_v ? 1 : 0;
   dev.real_v = real_v ? 1 : 0;
   dev.flat = d_flat.as<LinkFlat>();
+  dev.cmac_tab = d_cmac.as<z128>();
   dev.max_links = 0;
   for (int l = 0; l < hs.n_hrchy; ++l)
     dev.max_links = std::max(dev.max_links, link_ptr[l + 1] - link_ptr[l]);

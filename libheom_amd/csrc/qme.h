@@ -79,7 +79,7 @@ class Qme {
   // device copies
   FormDev dev{};
   hipStream_t stream = nullptr;
-  DevBuf d_link_ptr, d_links, d_flat, d_coef, d_gam, d_H, d_V, d_Vd, d_dterm, d_sdelta;
+  DevBuf d_link_ptr, d_links, d_flat, d_cmac, d_coef, d_gam, d_H, d_V, d_Vd, d_dterm, d_sdelta;
   DevBuf d_sop_ptr, d_sop_col, d_sop_val;
   DevBuf d_R_ptr, d_R_col, d_R_val;
   long long ado_nnz = 0;

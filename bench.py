@@ -139,7 +139,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "hilb_kernel<7,double,diag V> (fused Dormand-Prince stage pass)",
+            "kernel": "hilb_fast_kernel<7,double> (fused Dormand-Prince stage pass)",
             "bytes_per_launch": avg_bytes, "avg_launch_ms": avg_ms,
             "stage_ms": [float(x) for x in ms[:n]],
         },

@@ -1,0 +1,106 @@
+// heom_run: a reference-style C++ caller of the drop-in headers.
+//
+// Reads one model from a text file, fills the qme_base members exactly as a
+// libheom caller does (n_level, n_level_2, H, alloc_noises, V/len_gamma/gamma/
+// phi_0/sigma/S/A/s_delta, then set_param), propagates with qme_solver::solve
+// and prints rho_S at every output time (what the callback sees).
+//
+// Input (whitespace separated, complex numbers as "re im"):
+//   form(0 hilb|1 liou|2 ado) N max_depth n_noise
+//   H[N*N]
+//   per noise: len V[N*N] gamma[len*len] phi_0[len] sigma[len] S[len*len] A[len*len] s_delta
+//   solver(0 rk4|1 rkdp) dt atol rtol n_t t_list[n_t]
+//   rho0[main_size]
+// Output: one line per output time: t followed by rho_S (N*N complex, re im).
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <vector>
+
+#include "libheom_amd.hpp"
+
+using namespace libheom;
+using dtype = complex128;
+using engine = hip;
+
+static dtype rd(std::istream& in) {
+  double re, im;
+  in >> re >> im;
+  return dtype(re, im);
+}
+
+static void fill(std::istream& in, lil_matrix<dtype>& m, int rows, int cols) {
+  m.set_shape(rows, cols);
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j) {
+      dtype v = rd(in);
+      if (v != dtype(0.0)) m.push(i, j, v);
+    }
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: heom_run model.txt\n");
+    return 2;
+  }
+  std::ifstream in(argv[1]);
+  int form, N, depth, n_noise;
+  in >> form >> N >> depth >> n_noise;
+  std::unique_ptr<qme_base<dtype, row_major, engine>> qme;
+  if (form == 0)
+    qme.reset(new heom_hilb<dynamic, dtype, dense_matrix, row_major, engine>(depth, 1, 1));
+  else if (form == 1)
+    qme.reset(new heom_liou<dynamic, dtype, sparse_matrix, row_major, row_major, engine>(depth, 1, 1));
+  else
+    qme.reset(new heom_ado<dynamic, dtype, sparse_matrix, row_major, row_major, engine>(depth, 1, 1));
+  qme->n_level = N;
+  qme->n_level_2 = N * N;
+  fill(in, qme->H, N, N);
+  qme->alloc_noises(n_noise);
+  for (int u = 0; u < n_noise; ++u) {
+    int len;
+    in >> len;
+    fill(in, qme->V[u], N, N);
+    qme->len_gamma[u] = len;
+    fill(in, qme->gamma[u], len, len);
+    qme->phi_0[u].resize(len);
+    for (auto& x : qme->phi_0[u]) x = rd(in);
+    qme->sigma[u].resize(len);
+    for (auto& x : qme->sigma[u]) x = rd(in);
+    fill(in, qme->S[u], len, len);
+    fill(in, qme->A[u], len, len);
+    qme->s_delta[u] = rd(in);
+  }
+  int solver_kind, n_t;
+  double dt, atol, rtol;
+  in >> solver_kind >> dt >> atol >> rtol >> n_t;
+  std::vector<double> t_list(n_t);
+  for (auto& t : t_list) in >> t;
+
+  engine eng(0);
+  try {
+    qme->set_param(&eng);
+    std::vector<dtype> rho(qme->main_size());
+    for (auto& x : rho) x = rd(in);
+    std::unique_ptr<solver_base<dtype, row_major, engine>> solver;
+    if (solver_kind == 0)
+      solver.reset(new rk4<dtype, row_major, engine>());
+    else
+      solver.reset(new rkdp<dtype, row_major, engine>());
+    qme_solver<dtype, row_major, engine> qs(&eng, qme.get(), solver.get());
+    kwargs_t kw{{"dt", dt}, {"atol", atol}, {"rtol", rtol}};
+    qs.solve(rho.data(), t_list.data(), n_t,
+             [&](double t) {
+               std::printf("%.17g", t);
+               for (int k = 0; k < N * N; ++k)
+                 std::printf(" %.17g %.17g", rho[k].real(), rho[k].imag());
+               std::printf("\n");
+             },
+             kw);
+  } catch (heom_error& e) {
+    std::fprintf(stderr, "heom_error %d: %s\n", e.code, e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -487,6 +487,102 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
 }
 
 // ---------------------------------------------------------------------------
+// heom_hilb, large N (N > 16), diagonal V: one ADO per workgroup of 256 lanes,
+// each lane owns a 4x4 tile of elements.  The block sits in LDS (and H too
+// when both fit in 64 KB); the commutator is a register-tiled complex matrix
+// product, the neighbour terms and the stage epilogue run per owned element.
+// ---------------------------------------------------------------------------
+constexpr int kBigBS = 256;
+constexpr int kBigTile = 4;
+
+template <typename R, bool HLDS, bool F64>
+__global__ void __launch_bounds__(kBigBS) hilb_big_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  const int N = d.N, N2 = d.N2;
+  C* s_rho = reinterpret_cast<C*>(s_dyn);
+  C* s_H = s_rho + N2;
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  const auto link_ptr = cst(d.link_ptr);
+  const auto flat = cst(d.flat);
+  const auto gam = cst(d.gam);
+  const int nt = (N + kBigTile - 1) / kBigTile;  // tiles per row
+  if (HLDS)
+    for (int x = threadIdx.x; x < N2; x += kBigBS) s_H[x] = conv<R>(d.H[x]);
+  double err_acc = 0.0;
+  for (int g = blockIdx.x; g < d.n_hrchy; g += gridDim.x) {
+    const long long base = static_cast<long long>(g) * N2;
+    for (int x = threadIdx.x; x < N2; x += kBigBS) s_rho[x] = in[base + x];
+    __syncthreads();
+    const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
+    const C gm = conv<R>(z128{gam[g].re, gam[g].im});
+    for (int t = threadIdx.x; t < nt * nt; t += kBigBS) {
+      const int r0 = (t / nt) * kBigTile, c0 = (t % nt) * kBigTile;
+      C acc[kBigTile][kBigTile];
+#pragma unroll
+      for (int a = 0; a < kBigTile; ++a)
+#pragma unroll
+        for (int b = 0; b < kBigTile; ++b) acc[a][b] = C{0, 0};
+      for (int k = 0; k < N; ++k) {
+        C hr[kBigTile], rr[kBigTile], rc[kBigTile], hc[kBigTile];
+#pragma unroll
+        for (int a = 0; a < kBigTile; ++a) {
+          const int i = r0 + a < N ? r0 + a : N - 1;
+          const int j = c0 + a < N ? c0 + a : N - 1;
+          hr[a] = HLDS ? s_H[i * N + k] : conv<R>(d.H[i * N + k]);
+          rc[a] = s_rho[i * N + k];
+          rr[a] = s_rho[k * N + j];
+          hc[a] = HLDS ? s_H[k * N + j] : conv<R>(d.H[k * N + j]);
+        }
+#pragma unroll
+        for (int a = 0; a < kBigTile; ++a)
+#pragma unroll
+          for (int b = 0; b < kBigTile; ++b) {
+            cmac(acc[a][b], hr[a], rr[b]);
+            cmac(acc[a][b], C{-rc[a].re, -rc[a].im}, hc[b]);
+          }
+      }
+#pragma unroll
+      for (int a = 0; a < kBigTile; ++a)
+#pragma unroll
+        for (int b = 0; b < kBigTile; ++b) {
+          const int i = r0 + a, j = c0 + b;
+          if (i >= N || j >= N) continue;
+          const int e = i * N + j;
+          const C r = s_rho[e];
+          C D = times_i(acc[a][b]);
+          cmac(D, gm + conv<R>(d.dterm[e]), r);
+          for (int l = l0; l < l1; ++l) {
+            const LinkFlat q = ld_flat(flat + l);
+            const C x = in[static_cast<long long>(q.nbr) * N2 + e];
+            const long long tb = static_cast<long long>(q.u) * N2 + e;
+            C cf;
+            if (d.real_v) {
+              const z128 p = d.cmac_tab[tb];
+              cf.re = static_cast<R>(p.re) * static_cast<R>(q.iphi.re) +
+                      static_cast<R>(p.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
+              cf.im = static_cast<R>(p.re) * static_cast<R>(q.iphi.im) +
+                      static_cast<R>(p.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
+            } else {
+              cf = conv<R>(d.cmac_tab[2 * tb]) * conv<R>(q.iphi) +
+                   conv<R>(d.cmac_tab[2 * tb + 1]) * conv<R>(q.psi) + conv<R>(q.dir);
+            }
+            cmac(D, cf, x);
+          }
+          Pre pv;
+          prefetch<F64, kMaxPre>(pv, A, cur, base + e, true);
+          err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, base + e, D, pg.alpha);
+        }
+    }
+    __syncthreads();
+  }
+  if (A.do_err) write_partial(A, err_acc);
+}
+
+// ---------------------------------------------------------------------------
 // heom_liou: per-ADO superoperator form (heom_liou.h:151-255), N <= 16
 // ---------------------------------------------------------------------------
 template <typename R, bool F64>
@@ -657,6 +753,25 @@ int launch_hilb_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
 }
 
 template <typename R, bool F64>
+int launch_hilb_big(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  if (!d.diag_v)
+    throw HeomError{HEOM_ERR_UNSUPPORTED,
+                    "heom_hilb: n_level > 16 is provided for diagonal coupling operators V"};
+  const size_t one = sizeof(cx<R>) * static_cast<size_t>(d.N2);
+  const int grid = cap_grid(d.n_hrchy);
+  if (2 * one <= 64 * 1024) {
+    hipLaunchKernelGGL((hilb_big_kernel<R, true, F64>), dim3(grid), dim3(kBigBS), 2 * one, st, d,
+                       A);
+  } else if (one <= 64 * 1024) {
+    hipLaunchKernelGGL((hilb_big_kernel<R, false, F64>), dim3(grid), dim3(kBigBS), one, st, d,
+                       A);
+  } else {
+    throw HeomError{HEOM_ERR_UNSUPPORTED, "heom_hilb: one ADO block exceeds 64 KB of LDS"};
+  }
+  return grid;
+}
+
+template <typename R, bool F64>
 int launch_hilb(const FormDev& d, const StageArgs& A, hipStream_t st) {
   switch (d.N) {
 #define HAMD_CASE(n) \
@@ -667,7 +782,7 @@ int launch_hilb(const FormDev& d, const StageArgs& A, hipStream_t st) {
     HAMD_CASE(14) HAMD_CASE(15) HAMD_CASE(16)
 #undef HAMD_CASE
     default:
-      throw HeomError{HEOM_ERR_UNSUPPORTED, "heom_hilb: n_level > 16 not provided"};
+      return launch_hilb_big<R, F64>(d, A, st);
   }
 }
 

@@ -225,3 +225,55 @@ def test_rkdp_gives_up_on_nan_state():
     assert e.value.code == 4
     st = s.stats()
     assert st["gave_up"] == 1 and st["n_rejected"] == 1000
+
+
+def diag_model(N, n_noise, ln, L, seed, complex_diag=False):
+    m = configs.random_model(N, n_noise, ln, L, seed=seed, dense_v=False)
+    if complex_diag:
+        rng = np.random.default_rng(seed + 1)
+        for z in m["noises"]:
+            z["V"] = np.diag(rng.normal(size=N) + 1j * rng.normal(size=N))
+    return m
+
+
+@pytest.mark.parametrize("N", [17, 20, 33, 45, 46, 64])
+@pytest.mark.parametrize("complex_diag", [False, True])
+def test_calc_diff_large_n(N, complex_diag):
+    """heom_hilb for N > 16 (register-tiled large-N kernel), diagonal V."""
+    print("large N", N, complex_diag, flush=True)
+    m = diag_model(N, 2, 1, 2, seed=N, complex_diag=complex_diag)
+    g, o = pair("hilb", m)
+    rng = np.random.default_rng(N)
+    d0 = rng.normal(size=g.main_size()) + 1j * rng.normal(size=g.main_size())
+    for alpha, beta in [(1.0, 0.0), (0.5 - 0.25j, 0.3 + 0.2j)]:
+        ref = o.calc_diff(m["rho0"], alpha, beta, d0)
+        got = g.calc_diff_impl(None, d0.copy(), m["rho0"], alpha, beta)
+        assert np.abs(got - ref).max() <= TOL * max(1.0, np.abs(ref).max())
+
+
+def test_large_n_complex64_calc_diff():
+    m = diag_model(24, 2, 2, 2, seed=3)
+    g = H.heom_hilb(2, dtype="complex64").load(m, eng())
+    o = O.Oracle(O.HILB, 24, 2).load(m)
+    ref = o.calc_diff(m["rho0"], 1.0, 0.0)
+    got = g.calc_diff_impl(None, np.zeros_like(ref), m["rho0"], 1.0, 0.0)
+    assert np.abs(got - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_large_n_rkdp_parity():
+    m = diag_model(20, 2, 1, 2, seed=21)
+    m["rho0"] = m["rho0"] * 0.0
+    m["rho0"][0] = 1.0
+    m["t_list"] = np.linspace(0.0, 0.5, 3)
+    kw = {"dt": 0.01, "atol": 1e-10, "rtol": 1e-8}
+    rho, seen, rho_o, rs_o, st_o, st = _solve_both("hilb", m, "rkdp", kw)
+    assert st["n_accepted"] == st_o.n_accepted
+    assert np.abs(rho - rho_o).max() <= TOL
+
+
+def test_large_n_dense_v_is_reported_unsupported():
+    m = configs.random_model(17, 1, 1, 1, seed=2, dense_v=True)
+    g = H.heom_hilb(1).load(m, eng())
+    with pytest.raises(H.HeomError) as e:
+        g.calc_diff_impl(None, np.zeros(g.main_size(), np.complex128), m["rho0"], 1.0, 0.0)
+    assert e.value.code == 5

@@ -80,6 +80,8 @@ def main():
     eng = H.hip(0)
     qme = H.heom_hilb(m["max_depth"]).load(m, eng)
     solver = H.qme_solver(eng, qme, H.rkdp())
+    qme32 = H.heom_hilb(m["max_depth"], dtype="complex64").load(m, eng)
+    solver32 = H.qme_solver(eng, qme32, H.rkdp())
     L = _capi.load()
     kw = (m["dt"], m["atol"], m["rtol"])
     t_list = np.ascontiguousarray(m["t_list"])
@@ -89,15 +91,15 @@ def main():
     rs = np.zeros((t_list.size, 7, 7), np.complex128)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
 
-    def one_step():
-        solver.reset()
+    def one_step(sv=solver, out_rs=rs):
+        sv.reset()
         _capi.check(L.heom_memcpy_h2d(d, p(m["rho0"]), nb))
         _capi.check(L.heom_device_sync())
         t0 = time.perf_counter()
-        _capi.check(L.heom_solve_dev(solver._s, d, p(t_list), t_list.size, kw[0], kw[1],
-                                     kw[2], p(rs)))
+        _capi.check(L.heom_solve_dev(sv._s, d, p(t_list), t_list.size, kw[0], kw[1],
+                                     kw[2], p(out_rs)))
         _capi.check(L.heom_device_sync())
-        return time.perf_counter() - t0, solver.stats()
+        return time.perf_counter() - t0, sv.stats()
 
     for _ in range(args.warmup):
         one_step()
@@ -109,6 +111,23 @@ def main():
         rhs += st["n_rhs"]
     total = sum(times)
     value = updates / total
+
+    # complex64 variant: FP32 operator and stage inputs, FP64 state, stage
+    # combination and error norm; its rho_S error is taken against the FP64 run
+    rs32 = np.zeros_like(rs)
+    for _ in range(args.warmup):
+        one_step(solver32, rs32)
+    t32, u32 = 0.0, 0
+    for _ in range(args.steps):
+        el, st = one_step(solver32, rs32)
+        t32 += el
+        u32 += st["ado_stage_updates"]
+    variant32 = {
+        "dtype": "f32 operator / f64 combination", "value": u32 / t32,
+        "ms_per_step": 1e3 * t32 / args.steps,
+        "rho_s_max_abs_err_vs_fp64": float(np.abs(rs32 - rs).max()),
+        "speedup_time_to_rho_s": total / t32,
+    }
 
     # dominant kernel: per-stage device time (HIP events) and algorithmic bytes
     ns = ctypes.c_int()
@@ -144,6 +163,7 @@ def main():
             "stage_ms": [float(x) for x in ms[:n]],
         },
         "cpu_baseline": None,
+        "variants": {"complex64": variant32},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m)

@@ -28,6 +28,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hilb_fast_kernel_7site_traffic.json")
+
+
+def summarize_pmc(fetch_csv, write_csv, state_bytes):
+    """HBM bytes per dispatch of the stage kernel from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch).  Each counter is scaled by what
+    it reads for the largest rocclr copy in the same run, a device-to-device
+    copy of one state buffer (state_bytes read, state_bytes written)."""
+    import csv
+    import statistics
+
+    def load(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        copies = [r for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        big = max(int(r["Grid_Size"]) for r in copies)
+        cal = statistics.median(float(r["Counter_Value"]) for r in copies
+                                if int(r["Grid_Size"]) == big)
+        stage = [float(r["Counter_Value"]) for r in rows if "hilb_fast_kernel" in r["Kernel_Name"]]
+        return statistics.mean(stage), cal, len(stage)
+
+    f, fcal, n1 = load(fetch_csv, "FETCH_SIZE")
+    w, wcal, n2 = load(write_csv, "WRITE_SIZE")
+    fs, ws = state_bytes / (fcal * 1024.0), state_bytes / (wcal * 1024.0)
+    return {"fetch_bytes": f * 1024.0 * fs, "write_bytes": w * 1024.0 * ws,
+            "traffic": f * 1024.0 * fs + w * 1024.0 * ws,
+            "fetch_scale": fs, "write_scale": ws, "dispatches": [n1, n2],
+            "fetch_kib_raw": f, "write_kib_raw": w}
 
 
 def cpu_baseline(m, seconds=12.0):
@@ -67,7 +94,17 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--t-final", type=float, default=1000.0,
                     help="propagation length in fs (profiling runs use a shorter one)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the HBM traffic of the stage kernel from two rocprofv3 "
+                         "--pmc counter_collection.csv files and exit")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        from libheom_amd import configs as _c
+        m0 = _c.exciton_chain7(t_final=10.0)
+        t = summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], m0["rho0"].nbytes)
+        json.dump(t, open(TRAFFIC_JSON, "w"), indent=1)
+        print(json.dumps(t))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not provided "
               "(see DESIGN.md for how the hierarchy would shard)", file=sys.stderr)
@@ -140,6 +177,11 @@ def main():
     avg_ms = float(ms[:n].mean())
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
     L.heom_dev_free(d)
+    # measured HBM bytes per stage dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # calibrated on a copy of known size), recorded by --summarize-pmc
+    traffic = None
+    if os.path.exists(TRAFFIC_JSON):
+        traffic = json.load(open(TRAFFIC_JSON)).get("traffic")
 
     out = {
         "metric": "ADO-stage updates/s + time-to-ρ_S(t) at err≤1e-6 vs plain FP64 on 1 GPU",
@@ -157,7 +199,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "hilb_fast_kernel<7,double> (fused Dormand-Prince stage pass)",
             "bytes_per_launch": avg_bytes, "avg_launch_ms": avg_ms,
             "stage_ms": [float(x) for x in ms[:n]],

@@ -111,6 +111,20 @@ int heom_calc_diff(heom_handle h, void* drho_dev, const void* rho_dev, double al
 int heom_calc_diff_host(heom_handle h, double* drho, const double* rho, double alpha_re,
                         double alpha_im, double beta_re, double beta_im);
 
+/* Batch of independent heom_hilb members (diagonal V, n_level <= 8; no
+ * counterpart in the reference, which runs one hierarchy per object): the
+ * members' tables are packed so that one operator launch per Runge-Kutta stage
+ * advances all of them.  The batch behaves as one QME whose state is the
+ * concatenation of the members' states (member order); members must have had
+ * set_param; their tables are copied, so they may be destroyed afterwards.
+ * For a batch, heom_solve_dev's rho_s_out receives n_t blocks of all members'
+ * rho_S (member order) and the heom_solve callback sees member 0's rho_S. */
+int heom_batch_create(const heom_handle* members, int n_members, heom_handle* out);
+/* per member: first state element, offset of its rho_S in a gathered block,
+ * n_level (any array may be NULL) */
+int heom_batch_layout(heom_handle batch, long long* elem_base, int* rho_s_offset,
+                      int* n_level);
+
 /* solver_base::init via qme_solver (qme_solver.h:26-35) */
 int heom_solver_create(heom_handle h, int solver_kind, heom_solver_handle* out);
 int heom_solver_destroy(heom_solver_handle s);

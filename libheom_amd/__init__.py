@@ -28,7 +28,8 @@ import numpy as np
 from . import _capi
 from ._capi import HeomError
 
-__all__ = ["hip", "heom_hilb", "heom_liou", "heom_ado", "rk4", "rkdp", "qme_solver",
+__all__ = ["hip", "heom_hilb", "heom_liou", "heom_ado", "heom_batch", "rk4", "rkdp",
+           "qme_solver",
            "HeomError", "row_major", "col_major"]
 
 row_major = True
@@ -206,6 +207,79 @@ class heom_ado(_heom):
     _kind = _capi.HEOM_ADO
 
 
+class heom_batch:
+    """Independent heom_hilb hierarchies advanced together (no reference counterpart).
+
+    Members need diagonal coupling operators and n_level <= 8.  The batch state
+    is the concatenation of the members' states; ``pack`` / ``unpack`` convert
+    between the two.  One operator launch per Runge-Kutta stage covers every
+    member, so a parameter sweep of small systems fills the device.
+    """
+
+    def __init__(self, members):
+        members = list(members)
+        for q in members:
+            if not getattr(q, "_ready", False):
+                raise HeomError(_capi.HEOM_ERR_STATE, "member without set_param")
+        L = _capi.load()
+        arr = (ctypes.c_void_p * len(members))(*[q._h.value for q in members])
+        h = ctypes.c_void_p()
+        _capi.check(L.heom_batch_create(arr, len(members), ctypes.byref(h)))
+        self._h = h
+        self._ready = True
+        self.members = members
+        n = len(members)
+        self.elem_base = np.zeros(n, np.int64)
+        self.rho_s_offset = np.zeros(n, np.int32)
+        self.n_levels = np.zeros(n, np.int32)
+        _capi.check(L.heom_batch_layout(h, _ptr(self.elem_base), _ptr(self.rho_s_offset),
+                                        _ptr(self.n_levels)))
+        m = ctypes.c_longlong()
+        _capi.check(L.heom_main_size(h, ctypes.byref(m)))
+        self._main_size = m.value
+        self.n_level = int(self.n_levels[0])
+        self.dtype = members[0].dtype
+        nh = ctypes.c_int()
+        _capi.check(L.heom_get_n_hrchy(h, ctypes.byref(nh)))
+        self.n_hrchy = nh.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _capi.load().heom_destroy(self._h)
+            self._h = None
+
+    def main_size(self):
+        return self._main_size
+
+    def rho_s_size(self):
+        return int(np.sum(self.n_levels.astype(np.int64) ** 2))
+
+    def pack(self, rhos):
+        out = np.zeros(self._main_size, np.complex128)
+        for m, r in enumerate(rhos):
+            r = np.asarray(r, np.complex128).reshape(-1)
+            out[self.elem_base[m]:self.elem_base[m] + r.size] = r
+        return out
+
+    def unpack(self, rho):
+        ends = list(self.elem_base[1:]) + [self._main_size]
+        return [np.asarray(rho)[b:e] for b, e in zip(self.elem_base, ends)]
+
+    def split_rho_s(self, block):
+        """(n_t, rho_s_size) gathered rho_S -> list of (n_t, N_m, N_m) per member."""
+        block = np.asarray(block).reshape(block.shape[0], -1)
+        res = []
+        for o, n in zip(self.rho_s_offset, self.n_levels):
+            res.append(block[:, o:o + n * n].reshape(-1, n, n))
+        return res
+
+    def calc_diff_impl(self, engine, drho_dt, rho, alpha, beta, temp=None):
+        return _heom.calc_diff_impl(self, engine, drho_dt, rho, alpha, beta, temp)
+
+    def calc_diff_dev(self, drho_dev, rho_dev, alpha, beta, stream=None):
+        return _heom.calc_diff_dev(self, drho_dev, rho_dev, alpha, beta, stream)
+
+
 class rk4:
     """Classic RK4 (solver/rk4.h); kwargs: dt."""
     _kind = _capi.HEOM_RK4
@@ -281,8 +355,11 @@ class qme_solver:
         L = _capi.load()
         rho = np.array(rho, dtype=np.complex128, copy=True).reshape(-1)
         t = np.ascontiguousarray(np.asarray(t_list, dtype=np.float64))
-        N = self.qme.n_level
-        out = np.zeros((t.size, N, N), np.complex128)
+        if isinstance(self.qme, heom_batch):
+            out = np.zeros((t.size, self.qme.rho_s_size()), np.complex128)
+        else:
+            N = self.qme.n_level
+            out = np.zeros((t.size, N, N), np.complex128)
         d = ctypes.c_void_p()
         nb = rho.nbytes
         _capi.check(L.heom_dev_alloc(ctypes.byref(d), nb))

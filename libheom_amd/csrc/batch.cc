@@ -1,0 +1,154 @@
+// Batches of independent Hilbert-form hierarchies (parameter sweeps): the
+// members' host tables are packed into one set of device arrays so that one
+// operator launch per Runge-Kutta stage advances every member.
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "kernels.h"
+#include "qme.h"
+
+namespace hamd {
+
+void Qme::set_param_batch(const std::vector<Qme*>& members) {
+  if (members.empty()) throw HeomError{HEOM_ERR_ARG, "batch: no members"};
+  for (size_t m = 0; m < members.size(); ++m) {
+    const Qme* q = members[m];
+    const std::string tag = "batch member " + std::to_string(m) + ": ";
+    if (!q || !q->ready) throw HeomError{HEOM_ERR_STATE, tag + "set_param has not been called"};
+    if (q->kind != HEOM_HILB) throw HeomError{HEOM_ERR_UNSUPPORTED, tag + "not heom_hilb"};
+    if (!q->diag_v)
+      throw HeomError{HEOM_ERR_UNSUPPORTED, tag + "coupling operators must be diagonal"};
+    if (q->N > 8) throw HeomError{HEOM_ERR_UNSUPPORTED, tag + "n_level > 8"};
+    if (q->dtype != dtype) throw HeomError{HEOM_ERR_ARG, tag + "dtype differs from the batch"};
+    if (q->opt.device != opt.device) throw HeomError{HEOM_ERR_ARG, tag + "device differs"};
+  }
+  kind = kBatch;
+  batch_inst.clear();
+  std::vector<int> lp{0};
+  std::vector<LinkFlat> fl;
+  std::vector<z128> gm, Hs, tab, dt;
+  std::vector<BatchItem> items;
+  long long elem = 0;
+  int total_ado = 0;
+  int rs = 0;
+  N = 0;
+  for (size_t m = 0; m < members.size(); ++m) {
+    const Qme& q = *members[m];
+    BatchInst I{};
+    I.N = q.N;
+    I.N2 = q.N2;
+    I.n_hrchy = q.hs.n_hrchy;
+    I.apw = 64 / q.N2;
+    I.ado_base = total_ado;
+    I.h_off = static_cast<int>(Hs.size());
+    I.tab_off = static_cast<int>(tab.size());
+    I.dterm_off = static_cast<int>(dt.size());
+    I.real_v = q.real_v ? 1 : 0;
+    I.elem_base = elem;
+    I.rs_off = rs;
+    rs += q.N2;
+    batch_inst.push_back(I);
+    const int base_link = lp.back();
+    for (int l = 0; l < q.hs.n_hrchy; ++l) lp.push_back(base_link + q.link_ptr[l + 1]);
+    fl.insert(fl.end(), q.flat_host.begin(), q.flat_host.end());
+    gm.insert(gm.end(), q.ngamma.begin(), q.ngamma.end());
+    Hs.insert(Hs.end(), q.H_host.begin(), q.H_host.end());
+    tab.insert(tab.end(), q.cmac_host.begin(), q.cmac_host.end());
+    dt.insert(dt.end(), q.dterm_host.begin(), q.dterm_host.end());
+    for (int a = 0; a < q.hs.n_hrchy; a += I.apw) items.push_back(BatchItem{static_cast<int>(m), a});
+    elem += q.main_size();
+    total_ado += q.hs.n_hrchy;
+    N = std::max(N, q.N);
+  }
+  if (elem > (1LL << 31) - 1) throw HeomError{HEOM_ERR_UNSUPPORTED, "batch: state exceeds 2^31"};
+  N2 = N * N;
+  batch_main_size = elem;
+  batch_rs_size = rs;
+  hs = Hierarchy{};
+  hs.n_hrchy = total_ado;
+  n_links = static_cast<long long>(fl.size());
+  link_ptr = lp;
+  links.clear();
+  coef.clear();
+  diag_v = true;
+  real_v = false;
+  HAMD_HIP(hipSetDevice(opt.device));
+  d_link_ptr.upload(lp);
+  d_flat.upload(fl);
+  d_gam.upload(gm);
+  d_H.upload(Hs);
+  d_cmac.upload(tab);
+  d_dterm.upload(dt);
+  d_inst.upload(batch_inst);
+  d_items.upload(items);
+  dev = FormDev{};
+  dev.kind = kBatch;
+  dev.N = N;
+  dev.N2 = N2;
+  dev.n_hrchy = total_ado;
+  dev.main_size = elem;
+  dev.link_ptr = d_link_ptr.as<int>();
+  dev.flat = d_flat.as<LinkFlat>();
+  dev.gam = d_gam.as<z128>();
+  dev.H = d_H.as<z128>();
+  dev.cmac_tab = d_cmac.as<z128>();
+  dev.dterm = d_dterm.as<z128>();
+  dev.diag_v = 1;
+  dev.n_inst = static_cast<int>(batch_inst.size());
+  dev.n_items = static_cast<int>(items.size());
+  dev.inst = d_inst.as<BatchInst>();
+  dev.items = d_items.as<BatchItem>();
+  dev.batch_grid = std::min(dev.n_items, kMaxGrid);
+  ready = true;
+  autotune_batch();
+}
+
+// Grid-size autotuner of the persistent batch kernel: time one derivative
+// evaluation of the whole batch for a few grid sizes and keep the fastest.
+void Qme::autotune_batch() {
+  const long long ms = main_size();
+  const size_t eb = dtype == HEOM_C64 ? sizeof(z64) : sizeof(z128);
+  DevBuf a, b;
+  a.alloc(eb * ms);
+  b.alloc(eb * ms);
+  HAMD_HIP(hipMemsetAsync(a.p, 0, eb * ms, stream));
+  StageArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.mode = MODE_DIFF;
+  A.in[0] = A.in[1] = a.p;
+  A.in_lo = dtype == HEOM_C64;
+  A.dst[0][0] = A.dst[0][1] = b.p;
+  A.dst_lo[0] = A.in_lo;
+  A.a_re = 1.0;
+  const int rb = dtype == HEOM_C64 ? 4 : 8;
+  std::vector<int> cands;
+  for (int g : {512, 1024, 2048, 4096, 8192, 16384})
+    if (g < dev.n_items) cands.push_back(g);
+  cands.push_back(std::min(dev.n_items, kMaxGrid));
+  hipEvent_t e0, e1;
+  HAMD_HIP(hipEventCreate(&e0));
+  HAMD_HIP(hipEventCreate(&e1));
+  float best = std::numeric_limits<float>::max();
+  int best_g = cands.back();
+  for (int g : cands) {
+    dev.batch_grid = g;
+    launch_operator(dev, rb, A, stream);  // warm-up
+    HAMD_HIP(hipEventRecord(e0, stream));
+    for (int r = 0; r < 5; ++r) launch_operator(dev, rb, A, stream);
+    HAMD_HIP(hipEventRecord(e1, stream));
+    HAMD_HIP(hipEventSynchronize(e1));
+    float ms_ = 0.f;
+    HAMD_HIP(hipEventElapsedTime(&ms_, e0, e1));
+    if (ms_ < best) {
+      best = ms_;
+      best_g = g;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev.batch_grid = best_g;
+}
+
+}  // namespace hamd

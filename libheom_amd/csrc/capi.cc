@@ -51,6 +51,14 @@ hamd::Qme& Q(heom_handle h) {
   return *h->q;
 }
 
+// handles whose parameters the caller may still change (not batches)
+hamd::Qme& Qcfg(heom_handle h) {
+  hamd::Qme& q = Q(h);
+  if (q.kind == hamd::kBatch)
+    throw hamd::HeomError{HEOM_ERR_STATE, "a batch handle has no parameters of its own"};
+  return q;
+}
+
 hamd::Qme& Qready(heom_handle h) {
   hamd::Qme& q = Q(h);
   if (!q.ready) throw hamd::HeomError{HEOM_ERR_STATE, "set_param has not been called"};
@@ -118,7 +126,7 @@ int heom_destroy(heom_handle h) {
 
 int heom_set_H(heom_handle h, const double* H) {
   return guard([&] {
-    hamd::Qme& q = Q(h);
+    hamd::Qme& q = Qcfg(h);
     if (!H) throw hamd::HeomError{HEOM_ERR_ARG, "heom_set_H: NULL"};
     q.H = load_c(H, q.N2);
     q.H_set = true;
@@ -129,7 +137,7 @@ int heom_set_H(heom_handle h, const double* H) {
 
 int heom_alloc_noises(heom_handle h, int n_noise) {
   return guard([&] {
-    hamd::Qme& q = Q(h);
+    hamd::Qme& q = Qcfg(h);
     if (n_noise < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_alloc_noises: n_noise < 1"};
     q.noise.assign(n_noise, hamd::NoiseParams());
     q.ready = false;
@@ -141,7 +149,7 @@ int heom_set_noise(heom_handle h, int u, const double* V, int len_gamma, const d
                    const double* phi_0, const double* sigma, const double* S, const double* A,
                    double s_delta_re, double s_delta_im) {
   return guard([&] {
-    hamd::Qme& q = Q(h);
+    hamd::Qme& q = Qcfg(h);
     if (u < 0 || u >= static_cast<int>(q.noise.size()))
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_set_noise: noise index out of range"};
     if (len_gamma < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_set_noise: len_gamma < 1"};
@@ -165,7 +173,7 @@ int heom_set_noise(heom_handle h, int u, const double* V, int len_gamma, const d
 
 int heom_set_param(heom_handle h) {
   return guard([&] {
-    Q(h).set_param();
+    Qcfg(h).set_param();
     return HEOM_OK;
   });
 }
@@ -273,6 +281,34 @@ int heom_calc_diff_host(heom_handle h, double* drho, const double* rho, double a
   });
 }
 
+int heom_batch_create(const heom_handle* members, int n_members, heom_handle* out) {
+  return guard([&] {
+    if (!out || !members || n_members < 1)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_batch_create: bad arguments"};
+    *out = nullptr;
+    std::vector<hamd::Qme*> ms;
+    for (int m = 0; m < n_members; ++m) ms.push_back(&Qready(members[m]));
+    auto h = std::make_unique<heom_qme>();
+    h->q = std::make_unique<hamd::Qme>(HEOM_HILB, ms[0]->dtype, 1, 0, ms[0]->opt);
+    h->q->set_param_batch(ms);
+    *out = h.release();
+    return HEOM_OK;
+  });
+}
+
+int heom_batch_layout(heom_handle batch, long long* elem_base, int* rho_s_offset, int* n_level) {
+  return guard([&] {
+    hamd::Qme& q = Qready(batch);
+    if (q.kind != hamd::kBatch) throw hamd::HeomError{HEOM_ERR_ARG, "not a batch handle"};
+    for (size_t m = 0; m < q.batch_inst.size(); ++m) {
+      if (elem_base) elem_base[m] = q.batch_inst[m].elem_base;
+      if (rho_s_offset) rho_s_offset[m] = q.batch_inst[m].rs_off;
+      if (n_level) n_level[m] = q.batch_inst[m].N;
+    }
+    return HEOM_OK;
+  });
+}
+
 int heom_solver_create(heom_handle h, int kind, heom_solver_handle* out) {
   return guard([&] {
     if (!out) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solver_create: out is NULL"};
@@ -299,13 +335,18 @@ int heom_solve_dev(heom_solver_handle s, void* rho_dev, const double* t_list, in
     if (!s || !s->s) throw hamd::HeomError{HEOM_ERR_ARG, "null solver"};
     if (!rho_dev || !t_list) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solve_dev: NULL"};
     hamd::Solver& sv = *s->s;
-    const int N2 = sv.q->N2;
+    const bool batch = sv.q->kind == hamd::kBatch;
+    const int N2 = batch ? sv.q->batch_rs_size : sv.q->N2;
     hamd::DevBuf rs;
     if (rho_s_out) rs.alloc(sizeof(hamd::z128) * N2 * static_cast<size_t>(n_t));
     auto out = [&](int i, const void* cur) {
-      if (rho_s_out)
-        HAMD_HIP(hipMemcpyAsync(static_cast<char*>(rs.p) + sizeof(hamd::z128) * N2 * i, cur,
-                                sizeof(hamd::z128) * N2, hipMemcpyDeviceToDevice, sv.q->stream));
+      if (!rho_s_out) return;
+      char* dst = static_cast<char*>(rs.p) + sizeof(hamd::z128) * N2 * i;
+      if (batch)
+        hamd::launch_gather_rho_s(sv.q->dev, dst, cur, sv.q->stream);
+      else
+        HAMD_HIP(hipMemcpyAsync(dst, cur, sizeof(hamd::z128) * N2, hipMemcpyDeviceToDevice,
+                                sv.q->stream));
     };
     int rc = sv.solve(rho_dev, t_list, n_t, dt, atol, rtol, out);
     if (rho_s_out)

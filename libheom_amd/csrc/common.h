@@ -141,9 +141,30 @@ struct alignas(16) LinkFlat {
   z128 iphi, psi, dir;
 };
 
+// One member of a batch of independent Hilbert-form hierarchies packed into
+// one set of device tables (Qme::set_param_batch).
+struct BatchInst {
+  int N, N2, n_hrchy;
+  int apw;        // ADOs per 64-lane wavefront (64 / N^2)
+  int ado_base;   // first index in the packed link_ptr / gam arrays
+  int h_off;      // H in the packed H array (z128 units)
+  int tab_off;    // (v_i - v_j, v_i + v_j) table (z128 units)
+  int dterm_off;  // delta table (z128 units)
+  int real_v;
+  int rs_off;     // offset of the member's rho_S in a gathered all-members block
+  long long elem_base;  // first element of the member's state
+};
+
+// one wavefront of work: `apw` ADOs of member `inst` starting at local ADO `ado`
+struct BatchItem {
+  int inst, ado;
+};
+
+constexpr int kBatch = 3;  // FormDev::kind of a packed batch
+
 // Operator tables of one QME in device memory.
 struct FormDev {
-  int kind;  // 0 hilb, 1 liou, 2 ado
+  int kind;  // 0 hilb, 1 liou, 2 ado, 3 batch of hilb
   int N, N2, n_hrchy, n_noise;
   long long main_size;
   const int* link_ptr;   // n_hrchy+1
@@ -171,6 +192,10 @@ struct FormDev {
   const long long* R_ptr;
   const int* R_col;
   const z128* R_val;
+  // batch
+  int n_inst, n_items, batch_grid;
+  const BatchInst* inst;
+  const BatchItem* items;
 };
 
 // ---------------------------------------------------------------------------

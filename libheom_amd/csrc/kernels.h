@@ -19,6 +19,10 @@ void launch_err_finalize(const StageArgs& A, int nparts, hipStream_t stream);
 // Set up the Dormand-Prince control block for one output interval.
 void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t stream);
 
+// rho_S of every member of a batch (d.kind == kBatch), gathered in member
+// order from an FP64 state
+void launch_gather_rho_s(const FormDev& d, void* dst, const void* src, hipStream_t stream);
+
 // dst (FP32 complex) <- src (FP64 complex), n elements
 void launch_narrow(void* dst, const void* src, long long n, hipStream_t stream);
 // dst <- src for n complex elements of the given slot type

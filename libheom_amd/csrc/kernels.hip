@@ -583,6 +583,86 @@ __global__ void __launch_bounds__(kBigBS) hilb_big_kernel(const FormDev d, const
 }
 
 // ---------------------------------------------------------------------------
+// batch of Hilbert-form hierarchies (diagonal V, N <= 8): one persistent
+// launch advances every member; a 64-lane wavefront takes one work item
+// (64/N^2 ADOs of one member)
+// ---------------------------------------------------------------------------
+template <typename R, bool F64>
+__global__ void __launch_bounds__(64) batch_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  __shared__ C s_rho[64];
+  __shared__ C s_H[64];
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  const int lane = threadIdx.x;
+  const auto items = cst(d.items);
+  const auto insts = cst(d.inst);
+  double err_acc = 0.0;
+  for (int it = blockIdx.x; it < d.n_items; it += gridDim.x) {
+    const int inst = items[it].inst, ado0 = items[it].ado;
+    const int N = insts[inst].N, N2 = insts[inst].N2, apw = insts[inst].apw;
+    const int nh = insts[inst].n_hrchy, abase = insts[inst].ado_base;
+    const int hoff = insts[inst].h_off, toff = insts[inst].tab_off;
+    const int doff = insts[inst].dterm_off, realv = insts[inst].real_v;
+    const long long ebase = insts[inst].elem_base;
+    const int la = lane / N2;
+    const int e = lane - la * N2;
+    const int i = e / N, j = e - (e / N) * N;
+    const int ado = ado0 + la;
+    const bool ok = la < apw && ado < nh;
+    const long long gi = ebase + static_cast<long long>(ok ? ado : ado0) * N2 + (ok ? e : 0);
+    const C r = in[gi];
+    Pre pv;
+    prefetch<F64, kMaxPre>(pv, A, cur, gi, ok);
+    s_H[lane] = lane < N2 ? conv<R>(d.H[hoff + lane]) : C{0, 0};
+    s_rho[lane] = ok ? r : C{0, 0};
+    __syncthreads();
+    C acc{0, 0};
+    if (ok) {
+      const C* b = s_rho + la * N2;
+      for (int k = 0; k < N; ++k) {
+        cmac(acc, s_H[i * N + k], b[k * N + j]);
+        const C t = b[i * N + k];
+        cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
+      }
+    }
+    C D = times_i(acc);
+    if (ok) {
+      cmac(D, conv<R>(d.gam[abase + ado]) + conv<R>(d.dterm[doff + e]), r);
+      const int l0 = d.link_ptr[abase + ado], l1 = d.link_ptr[abase + ado + 1];
+      for (int l = l0; l < l1; ++l) {
+        const LinkFlat q = d.flat[l];
+        const C x = in[ebase + static_cast<long long>(q.nbr) * N2 + e];
+        const int tb = q.u * N2 + e;
+        C cf;
+        if (realv) {
+          const z128 p = d.cmac_tab[toff + tb];
+          cf.re = static_cast<R>(p.re) * static_cast<R>(q.iphi.re) +
+                  static_cast<R>(p.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
+          cf.im = static_cast<R>(p.re) * static_cast<R>(q.iphi.im) +
+                  static_cast<R>(p.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
+        } else {
+          cf = conv<R>(d.cmac_tab[toff + 2 * tb]) * conv<R>(q.iphi) +
+               conv<R>(d.cmac_tab[toff + 2 * tb + 1]) * conv<R>(q.psi) + conv<R>(q.dir);
+        }
+        cmac(D, cf, x);
+      }
+      err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, gi, D, pg.alpha);
+    }
+    __syncthreads();
+  }
+  if (A.do_err) write_partial(A, err_acc);
+}
+
+// rho_S of every batch member, gathered into one block (member order)
+__global__ void gather_rho_s_kernel(z128* dst, const z128* src, const BatchInst* inst) {
+  const BatchInst I = inst[blockIdx.x];
+  for (int e = threadIdx.x; e < I.N2; e += blockDim.x) dst[I.rs_off + e] = src[I.elem_base + e];
+}
+
+// ---------------------------------------------------------------------------
 // heom_liou: per-ADO superoperator form (heom_liou.h:151-255), N <= 16
 // ---------------------------------------------------------------------------
 template <typename R, bool F64>
@@ -789,6 +869,11 @@ int launch_hilb(const FormDev& d, const StageArgs& A, hipStream_t st) {
 template <typename R, bool F64>
 int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
   if (d.kind == 0) return launch_hilb<R, F64>(d, A, st);
+  if (d.kind == kBatch) {
+    const int grid = d.batch_grid > 0 ? d.batch_grid : 1;
+    hipLaunchKernelGGL((batch_kernel<R, F64>), dim3(grid), dim3(64), 0, st, d, A);
+    return grid;
+  }
   if (d.kind == 1) {
     const int APB = 256 / d.N2;
     const int grid = cap_grid((d.n_hrchy + APB - 1) / APB);
@@ -824,6 +909,12 @@ void launch_err_finalize(const StageArgs& A, int nparts, hipStream_t stream) {
 
 void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t stream) {
   hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(64), 0, stream, c, t0, t_bound, dt);
+  HAMD_HIP(hipGetLastError());
+}
+
+void launch_gather_rho_s(const FormDev& d, void* dst, const void* src, hipStream_t stream) {
+  hipLaunchKernelGGL(gather_rho_s_kernel, dim3(d.n_inst), dim3(64), 0, stream,
+                     static_cast<z128*>(dst), static_cast<const z128*>(src), d.inst);
   HAMD_HIP(hipGetLastError());
 }
 

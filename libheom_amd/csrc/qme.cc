@@ -514,6 +514,8 @@ void Qme::upload() {
     flat[l] = f;
   }
   d_flat.upload(flat);
+  flat_host = flat;
+  n_links = static_cast<long long>(links.size());
   {
     // (v_i - v_j, v_i + v_j) per noise and element, one zero row for direct links
     std::vector<z128> tab;
@@ -534,6 +536,7 @@ void Qme::upload() {
           }
     }
     d_cmac.upload(tab);
+    cmac_host = tab;
   }
   d_coef.upload(coef);
   d_gam.upload(ngamma);
@@ -559,6 +562,8 @@ void Qme::upload() {
   d_V.upload(V);
   d_Vd.upload(Vd);
   d_dterm.upload(dterm);
+  dterm_host = dterm;
+  H_host = H_work_;
   d_sdelta.upload(sd);
   if (!sop_ptr_.empty()) {
     d_sop_ptr.upload(sop_ptr_);

@@ -85,7 +85,24 @@ class Qme {
   long long ado_nnz = 0;
 
   void set_param();
-  long long main_size() const { return static_cast<long long>(hs.n_hrchy) * N2; }
+  // pack already set-up Hilbert-form members (diagonal V, n_level <= 8) into
+  // one batch whose state is the concatenation of the members' states
+  void set_param_batch(const std::vector<Qme*>& members);
+  long long main_size() const {
+    return kind == kBatch ? batch_main_size : static_cast<long long>(hs.n_hrchy) * N2;
+  }
+
+  // host copies kept for packing into a batch
+  std::vector<LinkFlat> flat_host;
+  std::vector<z128> cmac_host, dterm_host, H_host;
+  long long n_links = 0;
+
+  // batch
+  std::vector<BatchInst> batch_inst;
+  long long batch_main_size = 0;
+  int batch_rs_size = 0;  // sum of the members' N^2
+  DevBuf d_inst, d_items;
+  void autotune_batch();
 
  private:
   // parameters after the col_major transform (identical for row_major)

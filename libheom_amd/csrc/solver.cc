@@ -367,8 +367,9 @@ void Solver::stage_bytes(std::vector<double>& out) {
   if (q->kind == HEOM_ADO) {
     tables = 20.0 * q->ado_nnz + 8.0 * (ms + 1);
   } else {
-    const double lrec = (q->kind == HEOM_HILB && q->diag_v) ? sizeof(LinkFlat) : sizeof(Link);
-    tables = lrec * q->links.size() + 4.0 * (q->hs.n_hrchy + 1) + 16.0 * q->hs.n_hrchy;
+    const double lrec =
+        ((q->kind == HEOM_HILB || q->kind == kBatch) && q->diag_v) ? sizeof(LinkFlat) : sizeof(Link);
+    tables = lrec * q->n_links + 4.0 * (q->hs.n_hrchy + 1) + 16.0 * q->hs.n_hrchy;
   }
   for (int s = 0; s < ns; ++s) {
     const StageSpec& a = specs[s];

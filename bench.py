@@ -85,12 +85,152 @@ def cpu_baseline(m, seconds=12.0):
     }
 
 
+def bench_sweep(H, L, steps, warmup, reps):
+    """512-instance parameter sweep (N in {2,4,8}, K 2..4, L 4..12), RK4
+    dt=0.005 x 4000 steps, advanced as one batch: complex64 operator with FP64
+    stage combination (the sweep's stated precision), the FP64 batch beside it
+    for the error, and a sample of members run one at a time for comparison."""
+    from libheom_amd import _capi, configs
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    ms = configs.batched_sweep()
+    t_list = np.ascontiguousarray(ms[0]["t_list"])
+    dt = ms[0]["dt"]
+    eng = H.hip(0)
+    res = {}
+    for dtype in ("complex64", "complex128"):
+        b = H.heom_batch([H.heom_hilb(m["max_depth"], dtype=dtype).load(m, eng) for m in ms])
+        sv = H.qme_solver(eng, b, H.rk4())
+        rho0 = b.pack([m["rho0"] for m in ms])
+        nb = rho0.nbytes
+        d = ctypes.c_void_p()
+        _capi.check(L.heom_dev_alloc(ctypes.byref(d), nb))
+        rs = np.zeros((t_list.size, b.rho_s_size()), np.complex128)
+
+        def run():
+            _capi.check(L.heom_memcpy_h2d(d, p(rho0), nb))
+            _capi.check(L.heom_device_sync())
+            t0 = time.perf_counter()
+            _capi.check(L.heom_solve_dev(sv._s, d, p(t_list), t_list.size, dt, 0.0, 0.0, p(rs)))
+            _capi.check(L.heom_device_sync())
+            return time.perf_counter() - t0
+
+        for _ in range(warmup):
+            run()
+        sv.reset()
+        el = sum(run() for _ in range(steps))
+        st = sv.stats()
+        ns = ctypes.c_int()
+        sms, sby = np.zeros(8), np.zeros(8)
+        _capi.check(L.heom_solver_time_stages(sv._s, reps, p(sms), ctypes.byref(ns)))
+        _capi.check(L.heom_solver_stage_bytes(sv._s, p(sby), ctypes.byref(ns)))
+        n = ns.value
+        gbs = float(sby[:n].mean()) / (float(sms[:n].mean()) * 1e-3) / 1e9
+        L.heom_dev_free(d)
+        res[dtype] = {"value": st["ado_stage_updates"] / el, "ms_per_step": 1e3 * el / steps,
+                      "stage_ms": float(sms[:n].mean()), "stage_bytes": float(sby[:n].mean()),
+                      "hbm_frac": gbs / HBM_PEAK_GBS, "rs": b.split_rho_s(rs),
+                      "n_hrchy": b.n_hrchy, "main_size": b.main_size()}
+    # members whose FP64 rho_S stays a bounded density matrix (|rho_ij| <= 1);
+    # the others grow without bound (RK4 at dt=0.005 is outside its stability
+    # region for their largest Pade frequency x depth) and carry no error figure
+    worst, n_div = 0.0, 0
+    for a, r in zip(res["complex64"]["rs"], res["complex128"]["rs"]):
+        if not np.isfinite(r).all() or np.abs(r).max() > 1.0 + 1e-6:
+            n_div += 1
+            continue
+        worst = max(worst, float(np.abs(a - r).max()))
+    # members one at a time (FP64, same solver): a sample, scaled to 512 by ADO count
+    sample = ms[::64]
+    t_one, ado_one = 0.0, 0
+    for m in sample:
+        q = H.heom_hilb(m["max_depth"]).load(m, eng)
+        sv = H.qme_solver(eng, q, H.rk4())
+        t0 = time.perf_counter()
+        sv.solve_rho_s(m["rho0"], t_list, {"dt": dt})
+        t_one += time.perf_counter() - t0
+        ado_one += q.n_hrchy
+    one_ms = 1e3 * t_one * res["complex128"]["n_hrchy"] / ado_one
+    f32, f64 = res["complex64"], res["complex128"]
+    return {
+        "workload": "512 independent heom_hilb instances (N in {2,4,8}, 1 Drude + 1-3 Pade, "
+                    "L 4..12), RK4 dt=0.005 x 4000 steps, rho_S every 100 steps, one batch",
+        "n_hrchy_total": f64["n_hrchy"], "main_size": f64["main_size"],
+        "dtype": "f32 operator / f64 combination", "value": f32["value"],
+        "ms_per_step": f32["ms_per_step"], "stage_ms": f32["stage_ms"],
+        "hbm_frac": f32["hbm_frac"],
+        "fp64": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
+                 "stage_ms": f64["stage_ms"], "hbm_frac": f64["hbm_frac"]},
+        "rho_s_max_abs_err_vs_fp64_bounded_members": worst,
+        "members_unbounded_under_rk4": n_div,
+        "fp64_one_at_a_time_est_ms": one_ms,
+        "batch_speedup_vs_one_at_a_time_fp64": one_ms / f64["ms_per_step"],
+    }
+
+
+def bench_dense64(H, L, reps):
+    """Large-N config: N=64 GOE Hamiltonian, 4 projector baths x (1 Drude + 3
+    Pade) = 16 modes, depth 3 (969 ADOs of 64x64), Dormand-Prince rtol=1e-6 to
+    t=20, FP64 and complex64 (FP32 operator, FP64 combination)."""
+    from libheom_amd import _capi, configs
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    m = configs.dense64()
+    eng = H.hip(0)
+    t_list = np.ascontiguousarray(m["t_list"])
+    kw = {"dt": m["dt"], "atol": m["atol"], "rtol": m["rtol"]}
+    res = {}
+    for dtype in ("complex128", "complex64", "complex64_bf16x3", "valu"):
+        # "valu": FP64 on the vector-ALU kernel (matrix_cores off), for comparison
+        q = H.heom_hilb(m["max_depth"], dtype="complex128" if dtype == "valu" else dtype,
+                        matrix_cores=dtype != "valu").load(m, eng)
+        sv = H.qme_solver(eng, q, H.rkdp())
+        sv.solve_rho_s(m["rho0"], t_list[:2], kw)  # warm-up
+        sv.reset()
+        t_run = t_list if dtype != "valu" else t_list[:3]  # stage timing only
+        t0 = time.perf_counter()
+        _, rs = sv.solve_rho_s(m["rho0"], t_run, kw)
+        el = time.perf_counter() - t0
+        st = sv.stats()
+        ns = ctypes.c_int()
+        sms, sby = np.zeros(8), np.zeros(8)
+        _capi.check(L.heom_solver_time_stages(sv._s, reps, p(sms), ctypes.byref(ns)))
+        _capi.check(L.heom_solver_stage_bytes(sv._s, p(sby), ctypes.byref(ns)))
+        n = ns.value
+        stage_ms = float(sms[:n].mean())
+        # real flops per ADO-stage of the commutator (two complex N^3 products; H real
+        # symmetric here, so 4 real N^3 GEMMs) -- the MFMA-relevant part
+        flops = q.n_hrchy * 4 * 2 * 64 ** 3
+        res[dtype] = {"value": st["ado_stage_updates"] / el, "time_to_rho_s_ms": 1e3 * el,
+                      "stage_ms": stage_ms, "hbm_frac": float(sby[:n].mean()) / (stage_ms * 1e-3)
+                      / 1e9 / HBM_PEAK_GBS,
+                      "commutator_tflops": flops / (stage_ms * 1e-3) / 1e12,
+                      "rhs_evals": st["n_rhs"], "rs": rs}
+    f64 = res["complex128"]
+
+    def variant(r):
+        return dict({k: v for k, v in r.items() if k != "rs"},
+                    rho_s_max_abs_err_vs_fp64=float(np.abs(r["rs"] - f64["rs"]).max()),
+                    speedup_time_to_rho_s=f64["time_to_rho_s_ms"] / r["time_to_rho_s_ms"])
+
+    return {
+        "workload": "N=64 GOE H, 4 diagonal-projector baths x (1 Drude + 3 Pade), depth 3, "
+                    "969 ADOs, Dormand-Prince rtol=1e-6 atol=1e-9, t=0..20",
+        "kernel": "hilb_mfma_kernel<64> (f64 / f32 16x16x4 or bf16 16x16x32 MFMA commutator, "
+                  "fused Dormand-Prince stage)",
+        "fp64": {k: v for k, v in f64.items() if k != "rs"},
+        "complex64": variant(res["complex64"]),
+        "complex64_bf16x3": variant(res["complex64_bf16x3"]),
+        "fp64_vector_alu_stage_ms": res["valu"]["stage_ms"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-dense", action="store_true")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--t-final", type=float, default=1000.0,
                     help="propagation length in fs (profiling runs use a shorter one)")
@@ -207,6 +347,10 @@ def main():
         "cpu_baseline": None,
         "variants": {"complex64": variant32},
     }
+    if not args.no_dense:
+        out["variants"]["dense64"] = bench_dense64(H, L, 5)
+    if not args.no_sweep:
+        out["variants"]["batched_sweep_512"] = bench_sweep(H, L, 1, 1, 5)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m)
     print(json.dumps(out))

@@ -39,6 +39,10 @@ extern "C" {
 #define HEOM_C128 0  /* FP64 complex everywhere */
 #define HEOM_C64 1   /* FP32 complex operator + stage inputs; FP64 stage
                         combination, state and error norm */
+#define HEOM_C64_BF16X3 2 /* as HEOM_C64, with the commutator's matrix products
+                             on bf16 matrix cores: each FP32 operand split into
+                             hi + lo bf16 parts, hi*hi + hi*lo + lo*hi
+                             accumulated in FP32 (matrix-core path only) */
 
 /* solvers (solver/rk4.h, solver/rkdp.h) */
 #define HEOM_RK4 0
@@ -53,6 +57,8 @@ typedef struct heom_options {
   int sqrt_norm;  /* 1: LIBHEOM_SQRT_NORMALIZATION factors (heom_hilb.h:140-196) */
   int order;      /* 0: row_major (default), 1: col_major (matrix_base.h) */
   int device;     /* HIP device ordinal */
+  int matrix_cores; /* 1 (default): heom_hilb with n_level 32 or 64 and diagonal
+                       V computes the commutator on MFMA; 0: vector ALU kernels */
 } heom_options;
 
 typedef struct heom_stats {

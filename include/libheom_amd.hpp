@@ -153,6 +153,7 @@ class qme_base {
     opt.device = obj ? obj->device : 0;
     opt.parity_q1 = parity_q1;
     opt.sqrt_norm = sqrt_normalization;
+    opt.matrix_cores = matrix_cores;
     const int dt = std::is_same<dtype, complex64>::value ? HEOM_C64 : HEOM_C128;
     check(heom_create(kind_, dt, n_level, max_depth_, &opt, &h_));
     check(heom_set_H(h_, H.dense(n_level, n_level).data()));
@@ -188,6 +189,7 @@ class qme_base {
   // reference quirk switches (parity defaults)
   int parity_q1 = 1;
   int sqrt_normalization = 0;
+  int matrix_cores = 1;  // MFMA commutator where provided (n_level 32 / 64, diagonal V)
 
  protected:
   int kind_ = HEOM_HILB;

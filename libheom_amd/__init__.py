@@ -57,14 +57,20 @@ class _heom:
     _kind = None
 
     def __init__(self, max_depth, n_inner_threads=1, n_outer_threads=1, dtype="complex128",
-                 order=row_major, parity_q1=True, sqrt_normalization=False):
+                 order=row_major, parity_q1=True, sqrt_normalization=False, matrix_cores=True):
         # heom.h:44-50; thread counts are accepted for API parity, the device replaces them
         self.max_depth = int(max_depth)
         self.n_inner_threads = n_inner_threads
         self.n_outer_threads = n_outer_threads
+        # "complex64_bf16x3": complex64 state, commutator products on bf16 matrix
+        # cores with split (hi + lo) operands
+        self.bf16x3 = dtype == "complex64_bf16x3"
+        if self.bf16x3:
+            dtype = "complex64"
         if dtype not in ("complex128", "complex64", np.complex128, np.complex64):
-            raise ValueError("dtype must be complex128 or complex64")
+            raise ValueError("dtype must be complex128, complex64 or complex64_bf16x3")
         self.dtype = np.dtype(dtype)
+        self.matrix_cores = matrix_cores
         self.order = order
         self.parity_q1 = parity_q1
         self.sqrt_normalization = sqrt_normalization
@@ -103,9 +109,12 @@ class _heom:
             L.heom_destroy(self._h)
             self._h = None
         opt = _capi.heom_options(int(self.parity_q1), int(self.sqrt_normalization),
-                                 0 if self.order == row_major else 1, dev)
+                                 0 if self.order == row_major else 1, dev,
+                                 int(bool(self.matrix_cores)))
         h = ctypes.c_void_p()
         dt = _capi.HEOM_C64 if self.dtype == np.complex64 else _capi.HEOM_C128
+        if self.bf16x3:
+            dt = _capi.HEOM_C64_BF16X3
         _capi.check(L.heom_create(self._kind, dt, N, self.max_depth, ctypes.byref(opt),
                                   ctypes.byref(h)))
         self._h = h

@@ -19,7 +19,7 @@ HEOM_ERR_UNSUPPORTED = 5
 HEOM_ERR_NOMEM = 6
 
 HEOM_HILB, HEOM_LIOU, HEOM_ADO = 0, 1, 2
-HEOM_C128, HEOM_C64 = 0, 1
+HEOM_C128, HEOM_C64, HEOM_C64_BF16X3 = 0, 1, 2
 HEOM_RK4, HEOM_RKDP = 0, 1
 
 # every symbol declared in include/libheom_amd.h
@@ -44,7 +44,8 @@ class HeomError(RuntimeError):
 
 class heom_options(ctypes.Structure):
     _fields_ = [("parity_q1", ctypes.c_int), ("sqrt_norm", ctypes.c_int),
-                ("order", ctypes.c_int), ("device", ctypes.c_int)]
+                ("order", ctypes.c_int), ("device", ctypes.c_int),
+                ("matrix_cores", ctypes.c_int)]
 
 
 class heom_stats(ctypes.Structure):

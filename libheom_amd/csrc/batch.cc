@@ -109,7 +109,7 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
 // evaluation of the whole batch for a few grid sizes and keep the fastest.
 void Qme::autotune_batch() {
   const long long ms = main_size();
-  const size_t eb = dtype == HEOM_C64 ? sizeof(z64) : sizeof(z128);
+  const size_t eb = lo() ? sizeof(z64) : sizeof(z128);
   DevBuf a, b;
   a.alloc(eb * ms);
   b.alloc(eb * ms);
@@ -118,11 +118,11 @@ void Qme::autotune_batch() {
   std::memset(&A, 0, sizeof(A));
   A.mode = MODE_DIFF;
   A.in[0] = A.in[1] = a.p;
-  A.in_lo = dtype == HEOM_C64;
+  A.in_lo = lo();
   A.dst[0][0] = A.dst[0][1] = b.p;
   A.dst_lo[0] = A.in_lo;
   A.a_re = 1.0;
-  const int rb = dtype == HEOM_C64 ? 4 : 8;
+  const int rb = lo() ? 4 : 8;
   std::vector<int> cands;
   for (int g : {512, 1024, 2048, 4096, 8192, 16384})
     if (g < dev.n_items) cands.push_back(g);

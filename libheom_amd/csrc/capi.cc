@@ -89,6 +89,7 @@ int heom_default_options(heom_options* opt) {
   opt->sqrt_norm = 0;
   opt->order = 0;
   opt->device = 0;
+  opt->matrix_cores = 1;
   return HEOM_OK;
 }
 
@@ -99,7 +100,7 @@ int heom_create(int kind, int dtype, int n_level, int max_depth, const heom_opti
     *out = nullptr;
     if (kind < HEOM_HILB || kind > HEOM_ADO)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: unknown kind"};
-    if (dtype != HEOM_C128 && dtype != HEOM_C64)
+    if (dtype != HEOM_C128 && dtype != HEOM_C64 && dtype != HEOM_C64_BF16X3)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: unknown dtype"};
     if (n_level < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: n_level < 1"};
     if (max_depth < 0) throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: max_depth < 0"};
@@ -220,7 +221,7 @@ int heom_calc_diff(heom_handle h, void* drho_dev, const void* rho_dev, double al
     if (drho_dev == rho_dev)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_calc_diff: drho and rho must not alias"};
     HAMD_HIP(hipSetDevice(q.opt.device));
-    const int lo = q.dtype == HEOM_C64 ? 1 : 0;
+    const int lo = q.lo() ? 1 : 0;
     hamd::StageArgs A;
     std::memset(&A, 0, sizeof(A));
     A.mode = hamd::MODE_DIFF;
@@ -239,7 +240,7 @@ int heom_calc_diff(heom_handle h, void* drho_dev, const void* rho_dev, double al
     A.b_re = beta_re;
     A.b_im = beta_im;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : q.stream;
-    hamd::launch_operator(q.dev, q.dtype == HEOM_C64 ? 4 : 8, A, st);
+    hamd::launch_operator(q.dev, q.lo() ? 4 : 8, A, st);
     if (!stream) HAMD_HIP(hipStreamSynchronize(st));
     return HEOM_OK;
   });
@@ -251,7 +252,7 @@ int heom_calc_diff_host(heom_handle h, double* drho, const double* rho, double a
     hamd::Qme& q = Qready(h);
     if (!drho || !rho) throw hamd::HeomError{HEOM_ERR_ARG, "heom_calc_diff_host: NULL"};
     const long long ms = q.main_size();
-    const bool lo = q.dtype == HEOM_C64;
+    const bool lo = q.lo();
     const size_t eb = lo ? 8 : 16;
     hamd::DevBuf din, dout;
     din.alloc(eb * ms);

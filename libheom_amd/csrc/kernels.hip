@@ -15,6 +15,8 @@
 // layout) is read and written by contiguous lanes.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/libheom_amd.h"
 #include "common.h"
 #include "kernels.h"
@@ -583,6 +585,289 @@ __global__ void __launch_bounds__(kBigBS) hilb_big_kernel(const FormDev d, const
 }
 
 // ---------------------------------------------------------------------------
+// heom_hilb, N in {32, 64}, diagonal V: the commutator H*rho - rho*H on the
+// matrix cores (heom_hilb.h:108-125 computes it with two gemm calls per ADO).
+// One ADO per 256-thread workgroup.  rho_n is staged in LDS as separate real
+// and imaginary planes with row stride N+1, so that the 16 rows of an MFMA
+// fragment fall in distinct banks.  H never changes: each wave keeps its
+// 16-row band of H (the A operand of H*rho) in registers for the whole launch
+// and reads the B operand of rho*H from L2.  Wave w owns NB*NB/4 of the
+// (N/16)^2 output tiles, all in one 16-row band, and accumulates H*rho and
+// -rho*H into the same accumulators: 4 real products per k step when H is
+// real, 8 otherwise.  The accumulators then feed the element-wise part of
+// the operator and the fused Runge-Kutta epilogue.
+//
+// MFMA 16x16xK lane layout (checked against host products): lane l holds
+// A[l%16][k0 + l/16] and B[k0 + l/16][l%16] (K = 4), or the 8 consecutive k
+// from k0 + 8*(l/16) (bf16, K = 32); accumulator element r of lane l is
+// C[4*(l/16) + r][l%16] for the f32 accumulators and C[l/16 + 4*r][l%16]
+// for f64.
+// ---------------------------------------------------------------------------
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mf_b8 __attribute__((ext_vector_type(8)));
+
+template <typename R>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc = mf_d4;
+  static __device__ __forceinline__ acc mac(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+  static __device__ __forceinline__ const double* re(const FormDev& d) { return d.Hre; }
+  static __device__ __forceinline__ const double* im(const FormDev& d) { return d.Him; }
+};
+template <>
+struct Mfma<float> {
+  using acc = mf_f4;
+  static __device__ __forceinline__ acc mac(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+  static __device__ __forceinline__ const float* re(const FormDev& d) { return d.Hre32; }
+  static __device__ __forceinline__ const float* im(const FormDev& d) { return d.Him32; }
+};
+
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi): 16 significant bits
+struct Split8 {
+  mf_b8 hi, lo;
+};
+
+__device__ __forceinline__ Split8 split8(const float (&x)[8], bool neg) {
+  Split8 s;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const float v = neg ? -x[q] : x[q];
+    const __bf16 h = static_cast<__bf16>(v);
+    s.hi[q] = h;
+    s.lo[q] = static_cast<__bf16>(v - static_cast<float>(h));
+  }
+  return s;
+}
+
+// a*b ~ hi*hi + hi*lo + lo*hi (the lo*lo term, 2^-16 relative, is dropped);
+// the small terms are accumulated first
+__device__ __forceinline__ mf_f4 mac3(const Split8& a, const Split8& b, mf_f4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+}
+
+constexpr int kMfmaBS = 256;
+
+template <int N>
+struct MfmaCfg {
+  static constexpr int N2 = N * N;
+  static constexpr int LD = N + 1;                // LDS row stride
+  static constexpr int NB = N / 16;               // 16-row bands
+  static constexpr int TPW = NB * NB / 4;         // output tiles per wave
+  static constexpr int KS = N / 4;                // k steps of the 16x16x4 products
+  static constexpr int KB = N / 32;               // k steps of the bf16 products
+  static_assert(N == 32 || N == 64, "MFMA path: N in {32, 64}");
+};
+
+// H*rho - rho*H for the wave's tiles, operands in the state precision R
+template <int N, typename R, bool REALH>
+__device__ __forceinline__ void mfma_commutator(const FormDev& d, const R* s_re, const R* s_im,
+                                                const R (&ah_re)[MfmaCfg<N>::KS],
+                                                const R (&ah_im)[MfmaCfg<N>::KS], int band,
+                                                int t0, int lr, int lk,
+                                                typename Mfma<R>::acc (&cre)[MfmaCfg<N>::TPW],
+                                                typename Mfma<R>::acc (&cim)[MfmaCfg<N>::TPW]) {
+  using M = Mfma<R>;
+  using Cfg = MfmaCfg<N>;
+  const R* Hr = M::re(d);
+  const R* Hi = M::im(d);
+  const int row = band * 16 + lr;
+#pragma unroll
+  for (int ks = 0; ks < Cfg::KS; ++ks) {
+    const int kk = 4 * ks + lk;
+    const R arr = -s_re[row * Cfg::LD + kk], ari = -s_im[row * Cfg::LD + kk];
+    const R ahr = ah_re[ks], ahi = ah_im[ks];
+#pragma unroll
+    for (int t = 0; t < Cfg::TPW; ++t) {
+      const int col = ((t0 + t) % Cfg::NB) * 16 + lr;
+      const R brr = s_re[kk * Cfg::LD + col], bri = s_im[kk * Cfg::LD + col];
+      const R bhr = Hr[kk * N + col];
+      cre[t] = M::mac(ahr, brr, cre[t]);   //  Hr rho_r
+      cim[t] = M::mac(ahr, bri, cim[t]);   //  Hr rho_i
+      cre[t] = M::mac(arr, bhr, cre[t]);   // -rho_r Hr
+      cim[t] = M::mac(ari, bhr, cim[t]);   // -rho_i Hr
+      if (!REALH) {
+        const R bhi = Hi[kk * N + col];
+        cre[t] = M::mac(-ahi, bri, cre[t]);  // -Hi rho_i
+        cim[t] = M::mac(ahi, brr, cim[t]);   //  Hi rho_r
+        cre[t] = M::mac(-ari, bhi, cre[t]);  //  rho_i Hi
+        cim[t] = M::mac(arr, bhi, cim[t]);   // -rho_r Hi
+      }
+    }
+  }
+}
+
+// the same products on bf16 matrix cores with split operands (FP32 state)
+template <int N, bool REALH>
+__device__ __forceinline__ void mfma_commutator_bf16x3(const FormDev& d, const float* s_re,
+                                                       const float* s_im, int band, int t0,
+                                                       int lr, int lk,
+                                                       mf_f4 (&cre)[MfmaCfg<N>::TPW],
+                                                       mf_f4 (&cim)[MfmaCfg<N>::TPW]) {
+  using Cfg = MfmaCfg<N>;
+  const float* Hr = d.Hre32;
+  const float* Hi = d.Him32;
+  const int row = band * 16 + lr;
+  for (int kb = 0; kb < Cfg::KB; ++kb) {
+    const int k0 = 32 * kb + 8 * lk;
+    float x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = Hr[row * N + k0 + q];
+    const Split8 ahr = split8(x, false);
+    Split8 ahi{}, nahi{};
+    if (!REALH) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x[q] = Hi[row * N + k0 + q];
+      ahi = split8(x, false);
+      nahi = split8(x, true);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = s_re[row * Cfg::LD + k0 + q];
+    const Split8 arr = split8(x, true);   // -rho_r
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = s_im[row * Cfg::LD + k0 + q];
+    const Split8 ari = split8(x, true);   // -rho_i
+    const Split8 pari = split8(x, false);
+#pragma unroll
+    for (int t = 0; t < Cfg::TPW; ++t) {
+      const int col = ((t0 + t) % Cfg::NB) * 16 + lr;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x[q] = s_re[(k0 + q) * Cfg::LD + col];
+      const Split8 brr = split8(x, false);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x[q] = s_im[(k0 + q) * Cfg::LD + col];
+      const Split8 bri = split8(x, false);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x[q] = Hr[(k0 + q) * N + col];
+      const Split8 bhr = split8(x, false);
+      cre[t] = mac3(ahr, brr, cre[t]);
+      cim[t] = mac3(ahr, bri, cim[t]);
+      cre[t] = mac3(arr, bhr, cre[t]);
+      cim[t] = mac3(ari, bhr, cim[t]);
+      if (!REALH) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = Hi[(k0 + q) * N + col];
+        const Split8 bhi = split8(x, false);
+        cre[t] = mac3(nahi, bri, cre[t]);
+        cim[t] = mac3(ahi, brr, cim[t]);
+        cre[t] = mac3(pari, bhi, cre[t]);
+        cim[t] = mac3(arr, bhi, cim[t]);
+      }
+    }
+  }
+}
+
+// BF16X3: R = float, products via mfma_commutator_bf16x3
+template <int N, typename R, bool REALH, bool F64, int KP, bool BF16X3>
+__global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  using M = Mfma<R>;
+  using Cfg = MfmaCfg<N>;
+  constexpr int N2 = Cfg::N2, LD = Cfg::LD, TPW = Cfg::TPW, NB = Cfg::NB;
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  R* s_re = reinterpret_cast<R*>(s_dyn);
+  R* s_im = s_re + N * LD;
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  const auto link_ptr = cst(d.link_ptr);
+  const auto flat = cst(d.flat);
+  const auto gam = cst(d.gam);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int t0 = wave * TPW;
+  const int band = t0 / NB;
+  // the wave's band of H as MFMA A operands, kept for the whole launch
+  R ah_re[Cfg::KS], ah_im[Cfg::KS];
+  if (!BF16X3) {
+#pragma unroll
+    for (int ks = 0; ks < Cfg::KS; ++ks) {
+      const int e = (band * 16 + lr) * N + 4 * ks + lk;
+      ah_re[ks] = M::re(d)[e];
+      ah_im[ks] = REALH ? R(0) : M::im(d)[e];
+    }
+  }
+  double err_acc = 0.0;
+  for (int g = blockIdx.x; g < d.n_hrchy; g += gridDim.x) {
+    const long long base = static_cast<long long>(g) * N2;
+    for (int x = threadIdx.x; x < N2; x += kMfmaBS) {
+      const C v = in[base + x];
+      const int i = x / N, j = x % N;
+      s_re[i * LD + j] = v.re;
+      s_im[i * LD + j] = v.im;
+    }
+    __syncthreads();
+    typename M::acc cre[TPW], cim[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      cre[t] = typename M::acc{0, 0, 0, 0};
+      cim[t] = typename M::acc{0, 0, 0, 0};
+    }
+    if constexpr (BF16X3)
+      mfma_commutator_bf16x3<N, REALH>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
+    else
+      mfma_commutator<N, R, REALH>(d, s_re, s_im, ah_re, ah_im, band, t0, lr, lk, cre, cim);
+    // every wave has read rho_n as MFMA operands: each lane now replaces its
+    // own elements by i[H, rho] + (Gamma_n + delta term) rho (one owner each)
+    __syncthreads();
+    const C gm = conv<R>(z128{gam[g].re, gam[g].im});
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int col = ((t0 + t) % NB) * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = band * 16 + M::row(lk, r);
+        C D = times_i(C{cre[t][r], cim[t][r]});
+        const C rv{s_re[i * LD + col], s_im[i * LD + col]};
+        cmac(D, gm + conv<R>(d.dterm[i * N + col]), rv);
+        s_re[i * LD + col] = D.re;
+        s_im[i * LD + col] = D.im;
+      }
+    }
+    __syncthreads();
+    // neighbour couplings and the Runge-Kutta epilogue, element-parallel and
+    // coalesced
+    const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
+    for (int x = threadIdx.x; x < N2; x += kMfmaBS) {
+      const int i = x / N, j = x % N;
+      C D{s_re[i * LD + j], s_im[i * LD + j]};
+      PreT<KP> pv;
+      prefetch<F64, KP>(pv, A, cur, base + x, true);
+      for (int l = l0; l < l1; ++l) {
+        const LinkFlat q = ld_flat(flat + l);
+        const C xv = in[static_cast<long long>(q.nbr) * N2 + x];
+        const long long tb = static_cast<long long>(q.u) * N2 + x;
+        C cf;
+        if (d.real_v) {
+          const z128 p = d.cmac_tab[tb];
+          cf.re = static_cast<R>(p.re) * static_cast<R>(q.iphi.re) +
+                  static_cast<R>(p.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
+          cf.im = static_cast<R>(p.re) * static_cast<R>(q.iphi.im) +
+                  static_cast<R>(p.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
+        } else {
+          cf = conv<R>(d.cmac_tab[2 * tb]) * conv<R>(q.iphi) +
+               conv<R>(d.cmac_tab[2 * tb + 1]) * conv<R>(q.psi) + conv<R>(q.dir);
+        }
+        cmac(D, cf, xv);
+      }
+      err_acc += epilogue<F64, R, KP>(pv, A, cur, base + x, D, pg.alpha);
+    }
+    __syncthreads();
+  }
+  if (A.do_err) write_partial(A, err_acc);
+}
+
+// ---------------------------------------------------------------------------
 // batch of Hilbert-form hierarchies (diagonal V, N <= 8): one persistent
 // launch advances every member; a 64-lane wavefront takes one work item
 // (64/N^2 ADOs of one member)
@@ -832,8 +1117,53 @@ int launch_hilb_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
   return grid;
 }
 
+// dynamic LDS above 64 KB needs the per-kernel limit raised once
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    HAMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(bytes)));
+}
+
+template <int N, typename R, bool F64, bool BF16X3>
+int launch_hilb_mfma_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  const size_t lds = 2 * sizeof(R) * N * MfmaCfg<N>::LD;
+  const int grid = cap_grid(d.n_hrchy);
+  const bool small = A.npre <= 4 && !A.do_err;
+#define HAMD_MFMA(REALH, KP)                                                     \
+  {                                                                              \
+    auto k = hilb_mfma_kernel<N, R, REALH, F64, KP, BF16X3>;                     \
+    static bool once = (allow_lds(k, lds), true);                                \
+    (void)once;                                                                  \
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kMfmaBS), lds, st, d, A);             \
+  }
+  if (d.real_h) {
+    if (small) HAMD_MFMA(true, 4) else HAMD_MFMA(true, kMaxPre)
+  } else {
+    if (small) HAMD_MFMA(false, 4) else HAMD_MFMA(false, kMaxPre)
+  }
+#undef HAMD_MFMA
+  return grid;
+}
+
+template <typename R, bool F64>
+int launch_hilb_mfma(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  if (d.mfma == 2) {
+    if constexpr (std::is_same<R, float>::value) {
+      return d.N == 32 ? launch_hilb_mfma_n<32, float, false, true>(d, A, st)
+                       : launch_hilb_mfma_n<64, float, false, true>(d, A, st);
+    } else {
+      throw HeomError{HEOM_ERR_UNSUPPORTED, "bf16x3 products need the FP32 state"};
+    }
+  }
+  return d.N == 32 ? launch_hilb_mfma_n<32, R, F64, false>(d, A, st)
+                   : launch_hilb_mfma_n<64, R, F64, false>(d, A, st);
+}
+
 template <typename R, bool F64>
 int launch_hilb_big(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  if (d.mfma && (d.N == 32 || d.N == 64)) return launch_hilb_mfma<R, F64>(d, A, st);
   if (!d.diag_v)
     throw HeomError{HEOM_ERR_UNSUPPORTED,
                     "heom_hilb: n_level > 16 is provided for diagonal coupling operators V"};

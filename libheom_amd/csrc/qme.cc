@@ -606,6 +606,33 @@ void Qme::upload() {
   dev.R_ptr = d_R_ptr.as<long long>();
   dev.R_col = d_R_col.as<int>();
   dev.R_val = d_R_val.as<z128>();
+  // matrix-core commutator: heom_hilb, N in {32, 64}, diagonal V
+  const bool mfma_ok = kind == HEOM_HILB && diag_v && (N == 32 || N == 64);
+  dev.mfma = (mfma_ok && opt.matrix_cores) ? (dtype == HEOM_C64_BF16X3 ? 2 : 1) : 0;
+  if (dtype == HEOM_C64_BF16X3 && dev.mfma != 2)
+    throw HeomError{HEOM_ERR_UNSUPPORTED,
+                    "HEOM_C64_BF16X3 is provided for heom_hilb with n_level 32 or 64, diagonal "
+                    "coupling operators and matrix_cores = 1"};
+  dev.real_h = 1;
+  {
+    std::vector<double> hr(N2), hi(N2);
+    std::vector<float> hr32(N2), hi32(N2);
+    for (int e = 0; e < N2; ++e) {
+      hr[e] = H_work_[e].re;
+      hi[e] = H_work_[e].im;
+      hr32[e] = static_cast<float>(hr[e]);
+      hi32[e] = static_cast<float>(hi[e]);
+      if (hi[e] != 0.0) dev.real_h = 0;
+    }
+    d_Hre.upload(hr);
+    d_Him.upload(hi);
+    d_Hre32.upload(hr32);
+    d_Him32.upload(hi32);
+  }
+  dev.Hre = d_Hre.as<double>();
+  dev.Him = d_Him.as<double>();
+  dev.Hre32 = d_Hre32.as<float>();
+  dev.Him32 = d_Him32.as<float>();
 }
 
 }  // namespace hamd

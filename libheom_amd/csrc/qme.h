@@ -57,6 +57,8 @@ class Qme {
   ~Qme();
 
   int kind, dtype, N, N2, max_depth;
+  // FP32 operator / stage-input storage (HEOM_C64 and HEOM_C64_BF16X3)
+  bool lo() const { return dtype != HEOM_C128; }
   heom_options opt;
   std::vector<z128> H;
   bool H_set = false;
@@ -102,6 +104,7 @@ class Qme {
   long long batch_main_size = 0;
   int batch_rs_size = 0;  // sum of the members' N^2
   DevBuf d_inst, d_items;
+  DevBuf d_Hre, d_Him, d_Hre32, d_Him32;
   void autotune_batch();
 
  private:

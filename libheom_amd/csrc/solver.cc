@@ -64,7 +64,7 @@ Solver::Solver(Qme* q_, int kind_) : q(q_), kind(kind_) {
   if (kind != HEOM_RK4 && kind != HEOM_RKDP)
     throw HeomError{HEOM_ERR_ARG, "solver: unknown solver kind"};
   HAMD_HIP(hipSetDevice(q->opt.device));
-  in_bytes = q->dtype == HEOM_C64 ? 4 : 8;
+  in_bytes = q->lo() ? 4 : 8;
   ms = q->main_size();
   const size_t z128b = sizeof(z128) * ms;
   const size_t sinb = (in_bytes == 4 ? sizeof(z64) : sizeof(z128)) * ms;

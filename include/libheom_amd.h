@@ -131,6 +131,62 @@ int heom_batch_create(const heom_handle* members, int n_members, heom_handle* ou
 int heom_batch_layout(heom_handle batch, long long* elem_base, int* rho_s_offset,
                       int* n_level);
 
+/* Learned truncation closure (no counterpart in the reference, which drops
+ * every ADO beyond max_depth; heom.h:77-91).  A 3-layer network (n_in ->
+ * n_hidden -> n_hidden -> n_out, tanh, FP32) maps the ADOs of the last tier L
+ * to the tier-(L+1) coupling terms that truncation drops, added to their D_n.
+ * Row layouts:
+ *   HEOM_CLOSURE_PER_ADO: one row per last-tier ADO n = (Re rho_n, Im rho_n
+ *     (N^2 each), the index vector n, features) -> (Re, Im of n's term);
+ *   HEOM_CLOSURE_GLOBAL: one row per state = (Re, Im blocks of every
+ *     last-tier ADO in hierarchy order, features) -> all their terms.
+ * Training and inference run in the library's own kernels; X / Y are device
+ * arrays of rows (row-major FP32). */
+#define HEOM_CLOSURE_PER_ADO 0
+#define HEOM_CLOSURE_GLOBAL 1
+typedef struct heom_mlp* heom_mlp_handle;
+int heom_mlp_create(int n_in, int n_hidden, int n_out, unsigned long long seed, int device,
+                    heom_mlp_handle* out);
+int heom_mlp_destroy(heom_mlp_handle m);
+int heom_mlp_n_params(heom_mlp_handle m, long long* n_params);
+/* parameter order: W1 (n_hidden x n_in, row-major), b1, W2, b2, W3 (n_out x n_hidden), b3 */
+int heom_mlp_get_params(heom_mlp_handle m, float* host);
+int heom_mlp_set_params(heom_mlp_handle m, const float* host);
+/* x' = (x - x_shift) / x_scale, y = y' * y_scale + y_shift (host arrays) */
+int heom_mlp_set_scaling(heom_mlp_handle m, const float* x_shift, const float* x_scale,
+                         const float* y_shift, const float* y_scale);
+int heom_mlp_get_scaling(heom_mlp_handle m, float* x_shift, float* x_scale, float* y_shift,
+                         float* y_scale);
+/* per-column mean / standard deviation of the training rows */
+int heom_mlp_fit_scaling(heom_mlp_handle m, const float* X_dev, const float* Y_dev,
+                         long long n_rows);
+/* raw rows in, raw rows out (device, row-major) */
+int heom_mlp_forward(heom_mlp_handle m, const float* X_dev, float* Y_dev, int n_rows);
+/* mean squared error of the normalised outputs and its gradient (host array
+ * of n_params, may be NULL); the gradient also stays on the device for
+ * heom_mlp_adam_step */
+int heom_mlp_loss_grad(heom_mlp_handle m, const float* X_dev, const float* Y_dev, int n_rows,
+                       double* loss, float* grad_host);
+int heom_mlp_adam_step(heom_mlp_handle m, double lr, double beta1, double beta2, double eps);
+/* shuffled minibatch Adam (beta 0.9 / 0.999, eps 1e-8); epoch_loss[epochs] may be NULL */
+int heom_mlp_train(heom_mlp_handle m, const float* X_dev, const float* Y_dev, long long n_rows,
+                   int batch, int epochs, double lr, unsigned long long seed,
+                   double* epoch_loss);
+/* bind a snapshot of the network to heom_hilb h (m == NULL removes it); every
+ * later calc_diff / solve adds the predicted term to the last tier */
+int heom_set_closure(heom_handle h, heom_mlp_handle m, const float* features, int n_features,
+                     int layout);
+/* network rows per state and widths of a layout for heom_hilb h */
+int heom_closure_dims(heom_handle h, int layout, int n_features, int* rows_per_state, int* n_in,
+                      int* n_out);
+/* training rows from a state of a deeper hierarchy: shallow is the same model
+ * at depth L < depth(deep), both heom_hilb complex128; writes the rows of the
+ * last tier of shallow: x (n_in wide) and y = the tier-(L+1) term of D_n that
+ * truncation at L drops */
+int heom_closure_rows(heom_handle deep, heom_handle shallow, const void* rho_deep_dev,
+                      const float* features, int n_features, int layout, float* x_dev,
+                      float* y_dev);
+
 /* solver_base::init via qme_solver (qme_solver.h:26-35) */
 int heom_solver_create(heom_handle h, int solver_kind, heom_solver_handle* out);
 int heom_solver_destroy(heom_solver_handle s);

@@ -194,6 +194,13 @@ class _heom:
             drho_dt.reshape(-1)[:] = out
         return out
 
+    def set_closure(self, mlp, features=(), layout=_capi.HEOM_CLOSURE_PER_ADO):
+        """Bind a snapshot of a trained closure network (libheom_amd.closure.Mlp)
+        to this depth-L heom_hilb; None removes it."""
+        f = np.ascontiguousarray(features, np.float32)
+        _capi.check(_capi.load().heom_set_closure(self._h, mlp._h if mlp is not None else None,
+                                                  _ptr(f) if f.size else None, f.size, layout))
+
     def calc_diff_dev(self, drho_dev, rho_dev, alpha, beta, stream=None):
         """Device-pointer variant (heom_calc_diff)."""
         a, b = complex(alpha), complex(beta)

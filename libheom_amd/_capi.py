@@ -21,6 +21,7 @@ HEOM_ERR_NOMEM = 6
 HEOM_HILB, HEOM_LIOU, HEOM_ADO = 0, 1, 2
 HEOM_C128, HEOM_C64, HEOM_C64_BF16X3 = 0, 1, 2
 HEOM_RK4, HEOM_RKDP = 0, 1
+HEOM_CLOSURE_PER_ADO, HEOM_CLOSURE_GLOBAL = 0, 1
 
 # every symbol declared in include/libheom_amd.h
 EXPORTS = [
@@ -30,7 +31,11 @@ EXPORTS = [
     "heom_calc_diff_host", "heom_batch_create", "heom_batch_layout", "heom_solver_create", "heom_solver_destroy", "heom_solve",
     "heom_solve_dev", "heom_solver_stats", "heom_solver_reset", "heom_solver_time_stages",
     "heom_solver_stage_bytes", "heom_dev_alloc", "heom_dev_free", "heom_memcpy_h2d",
-    "heom_memcpy_d2h", "heom_device_sync",
+    "heom_memcpy_d2h", "heom_device_sync", "heom_mlp_create", "heom_mlp_destroy",
+    "heom_mlp_n_params", "heom_mlp_get_params", "heom_mlp_set_params", "heom_mlp_set_scaling",
+    "heom_mlp_get_scaling", "heom_mlp_fit_scaling", "heom_mlp_forward", "heom_mlp_loss_grad",
+    "heom_mlp_adam_step", "heom_mlp_train", "heom_set_closure", "heom_closure_dims",
+    "heom_closure_rows",
 ]
 
 
@@ -103,6 +108,21 @@ def load():
         "heom_memcpy_h2d": (ip, [vp, vp, llp]),
         "heom_memcpy_d2h": (ip, [vp, vp, llp]),
         "heom_device_sync": (ip, []),
+        "heom_mlp_create": (ip, [ip, ip, ip, ctypes.c_ulonglong, ip, P(vp)]),
+        "heom_mlp_destroy": (ip, [vp]),
+        "heom_mlp_n_params": (ip, [vp, P(llp)]),
+        "heom_mlp_get_params": (ip, [vp, vp]),
+        "heom_mlp_set_params": (ip, [vp, vp]),
+        "heom_mlp_set_scaling": (ip, [vp, vp, vp, vp, vp]),
+        "heom_mlp_get_scaling": (ip, [vp, vp, vp, vp, vp]),
+        "heom_mlp_fit_scaling": (ip, [vp, vp, vp, llp]),
+        "heom_mlp_forward": (ip, [vp, vp, vp, ip]),
+        "heom_mlp_loss_grad": (ip, [vp, vp, vp, ip, P(dp), vp]),
+        "heom_mlp_adam_step": (ip, [vp, dp, dp, dp, dp]),
+        "heom_mlp_train": (ip, [vp, vp, vp, llp, ip, ip, dp, ctypes.c_ulonglong, vp]),
+        "heom_set_closure": (ip, [vp, vp, vp, ip, ip]),
+        "heom_closure_dims": (ip, [vp, ip, ip, P(ip), P(ip), P(ip)]),
+        "heom_closure_rows": (ip, [vp, vp, vp, vp, ip, ip, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

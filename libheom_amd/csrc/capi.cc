@@ -8,10 +8,14 @@
 #include "../../include/libheom_amd.h"
 #include "kernels.h"
 #include "qme.h"
+#include "mlp.h"
 #include "solver.h"
 
 struct heom_qme {
   std::unique_ptr<hamd::Qme> q;
+};
+struct heom_mlp {
+  std::unique_ptr<hamd::Mlp> m;
 };
 struct heom_solver {
   std::unique_ptr<hamd::Solver> s;
@@ -57,6 +61,11 @@ hamd::Qme& Qcfg(heom_handle h) {
   if (q.kind == hamd::kBatch)
     throw hamd::HeomError{HEOM_ERR_STATE, "a batch handle has no parameters of its own"};
   return q;
+}
+
+hamd::Mlp& M(heom_mlp_handle m) {
+  if (!m || !m->m) throw hamd::HeomError{HEOM_ERR_ARG, "null network handle"};
+  return *m->m;
 }
 
 hamd::Qme& Qready(heom_handle h) {
@@ -221,26 +230,10 @@ int heom_calc_diff(heom_handle h, void* drho_dev, const void* rho_dev, double al
     if (drho_dev == rho_dev)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_calc_diff: drho and rho must not alias"};
     HAMD_HIP(hipSetDevice(q.opt.device));
-    const int lo = q.lo() ? 1 : 0;
-    hamd::StageArgs A;
-    std::memset(&A, 0, sizeof(A));
-    A.mode = hamd::MODE_DIFF;
-    A.in[0] = A.in[1] = rho_dev;
-    A.in_lo = lo;
-    A.dst[0][0] = A.dst[0][1] = drho_dev;
-    A.dst_lo[0] = lo;
-    A.ynew_pre = A.yold_pre = -1;
-    if (beta_re != 0.0 || beta_im != 0.0) {
-      A.npre = 1;
-      A.pre[0][0] = A.pre[1][0] = drho_dev;
-      A.pre_lo = lo ? 1u : 0u;
-    }
-    A.a_re = alpha_re;
-    A.a_im = alpha_im;
-    A.b_re = beta_re;
-    A.b_im = beta_im;
+    const hamd::StageArgs A = q.diff_args(drho_dev, rho_dev, hamd::z128{alpha_re, alpha_im},
+                                          hamd::z128{beta_re, beta_im});
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : q.stream;
-    hamd::launch_operator(q.dev, q.lo() ? 4 : 8, A, st);
+    q.operate(q.lo() ? 4 : 8, A, st);
     if (!stream) HAMD_HIP(hipStreamSynchronize(st));
     return HEOM_OK;
   });
@@ -306,6 +299,148 @@ int heom_batch_layout(heom_handle batch, long long* elem_base, int* rho_s_offset
       if (rho_s_offset) rho_s_offset[m] = q.batch_inst[m].rs_off;
       if (n_level) n_level[m] = q.batch_inst[m].N;
     }
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_create(int n_in, int n_hidden, int n_out, unsigned long long seed, int device,
+                    heom_mlp_handle* out) {
+  return guard([&] {
+    if (!out) throw hamd::HeomError{HEOM_ERR_ARG, "heom_mlp_create: out is NULL"};
+    *out = nullptr;
+    auto h = std::make_unique<heom_mlp>();
+    h->m = std::make_unique<hamd::Mlp>(n_in, n_hidden, n_out, seed, device);
+    *out = h.release();
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_destroy(heom_mlp_handle m) {
+  return guard([&] {
+    delete m;
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_n_params(heom_mlp_handle m, long long* n) {
+  return guard([&] {
+    if (!n) throw hamd::HeomError{HEOM_ERR_ARG, "NULL output"};
+    *n = M(m).n_params();
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_get_params(heom_mlp_handle m, float* host) {
+  return guard([&] {
+    if (!host) throw hamd::HeomError{HEOM_ERR_ARG, "NULL output"};
+    M(m).get_params(host);
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_set_params(heom_mlp_handle m, const float* host) {
+  return guard([&] {
+    if (!host) throw hamd::HeomError{HEOM_ERR_ARG, "NULL input"};
+    M(m).set_params(host);
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_set_scaling(heom_mlp_handle m, const float* xs, const float* xc, const float* ys,
+                         const float* yc) {
+  return guard([&] {
+    if (!xs || !xc || !ys || !yc) throw hamd::HeomError{HEOM_ERR_ARG, "NULL input"};
+    M(m).set_scaling(xs, xc, ys, yc);
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_get_scaling(heom_mlp_handle m, float* xs, float* xc, float* ys, float* yc) {
+  return guard([&] {
+    M(m).get_scaling(xs, xc, ys, yc);
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_fit_scaling(heom_mlp_handle m, const float* X, const float* Y, long long n_rows) {
+  return guard([&] {
+    if (!X || !Y) throw hamd::HeomError{HEOM_ERR_ARG, "NULL input"};
+    M(m).fit_scaling(X, Y, n_rows);
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_forward(heom_mlp_handle m, const float* X, float* Y, int n_rows) {
+  return guard([&] {
+    if (!X || !Y || n_rows < 0) throw hamd::HeomError{HEOM_ERR_ARG, "heom_mlp_forward: bad input"};
+    M(m).forward(X, Y, n_rows);
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_loss_grad(heom_mlp_handle m, const float* X, const float* Y, int n_rows,
+                       double* loss, float* grad_host) {
+  return guard([&] {
+    if (!X || !Y || n_rows < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_mlp_loss_grad: bad input"};
+    const double l = M(m).loss_grad(X, Y, n_rows, nullptr);
+    if (loss) *loss = l;
+    if (grad_host) M(m).get_grad(grad_host);
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_adam_step(heom_mlp_handle m, double lr, double beta1, double beta2, double eps) {
+  return guard([&] {
+    M(m).adam_step(lr, beta1, beta2, eps);
+    HAMD_HIP(hipStreamSynchronize(M(m).stream));
+    return HEOM_OK;
+  });
+}
+
+int heom_mlp_train(heom_mlp_handle m, const float* X, const float* Y, long long n_rows, int batch,
+                   int epochs, double lr, unsigned long long seed, double* epoch_loss) {
+  return guard([&] {
+    if (!X || !Y) throw hamd::HeomError{HEOM_ERR_ARG, "NULL input"};
+    M(m).train(X, Y, n_rows, batch, epochs, lr, seed, epoch_loss);
+    return HEOM_OK;
+  });
+}
+
+int heom_set_closure(heom_handle h, heom_mlp_handle m, const float* features, int n_features,
+                     int layout) {
+  return guard([&] {
+    hamd::Qme& q = Qready(h);
+    if (q.kind == hamd::kBatch)
+      throw hamd::HeomError{HEOM_ERR_UNSUPPORTED, "closure: not for batch handles"};
+    q.clear_closure();
+    if (m) {
+      auto c = std::make_unique<hamd::Closure>();
+      c->bind(q, M(m), features, n_features, layout);
+      q.closure = std::move(c);
+    }
+    return HEOM_OK;
+  });
+}
+
+int heom_closure_dims(heom_handle h, int layout, int n_features, int* rows, int* n_in,
+                      int* n_out) {
+  return guard([&] {
+    const hamd::ClosureDims c = hamd::closure_dims(Qready(h), layout, n_features);
+    if (rows) *rows = c.rows;
+    if (n_in) *n_in = c.n_in;
+    if (n_out) *n_out = c.n_out;
+    return HEOM_OK;
+  });
+}
+
+int heom_closure_rows(heom_handle deep, heom_handle shallow, const void* rho_deep_dev,
+                      const float* features, int n_features, int layout, float* x_dev,
+                      float* y_dev) {
+  return guard([&] {
+    if (!rho_deep_dev || !x_dev || !y_dev)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_closure_rows: NULL"};
+    hamd::closure_rows(Qready(deep), Qready(shallow), rho_deep_dev, features, n_features, layout,
+                       x_dev, y_dev);
     return HEOM_OK;
   });
 }

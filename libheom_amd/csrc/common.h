@@ -201,6 +201,11 @@ struct FormDev {
   const double* Him;
   const float* Hre32;
   const float* Him32;
+  // learned truncation closure: rows clos_lo .. clos_lo + clos_n - 1 of the
+  // ADOs (the last tier) get clos[(a - clos_lo) * 2 N^2 + e] (real part) and
+  // [... + N^2 + e] (imaginary part) added to D; null when no closure is bound
+  const float* clos;
+  int clos_lo, clos_n;
   // batch
   int n_inst, n_items, batch_grid;
   const BatchInst* inst;

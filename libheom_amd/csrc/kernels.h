@@ -21,6 +21,10 @@ void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t
 
 // rho_S of every member of a batch (d.kind == kBatch), gathered in member
 // order from an FP64 state
+// closure network input: Re / Im (N^2 each) of every last-tier ADO of the
+// stage input, ADO r's block starting at x[r * ado_stride]
+void launch_closure_gather(const FormDev& d, const StageArgs& A, float* x, int ado_stride,
+                           hipStream_t stream);
 void launch_gather_rho_s(const FormDev& d, void* dst, const void* src, hipStream_t stream);
 
 // dst (FP32 complex) <- src (FP64 complex), n elements

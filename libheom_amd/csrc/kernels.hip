@@ -247,6 +247,20 @@ __device__ __forceinline__ Prologue prologue(const StageArgs& A) {
   return p;
 }
 
+// Learned truncation closure: the network's prediction of the tier-(L+1)
+// coupling term of a last-tier ADO, added to D before the epilogue.
+template <typename R>
+__device__ __forceinline__ void add_closure(const FormDev& d, long long gi, cx<R>& D) {
+  if (d.clos == nullptr) return;
+  const long long a = gi / d.N2;
+  const int r = static_cast<int>(a) - d.clos_lo;
+  if (r < 0 || r >= d.clos_n) return;
+  const int e = static_cast<int>(gi - a * d.N2);
+  const float* y = d.clos + static_cast<long long>(r) * 2 * d.N2;
+  D.re += static_cast<R>(y[e]);
+  D.im += static_cast<R>(y[d.N2 + e]);
+}
+
 // ---------------------------------------------------------------------------
 // heom_hilb, fast path: one ADO per workgroup (N = 6..16), diagonal V
 // (heom_hilb.h:96-212 with [V,X]_ij = (v_i - v_j) X_ij, {V,X}_ij = (v_i + v_j) X_ij)
@@ -340,7 +354,10 @@ __global__ void __launch_bounds__(WaveCfg<N>::BS)
     }
   }
   double w2 = 0.0;
-  if (lane_ok) w2 = epilogue<F64, R, KP>(pv, A, cur, gi, D, pg.alpha);
+  if (lane_ok) {
+    add_closure(d, gi, D);
+    w2 = epilogue<F64, R, KP>(pv, A, cur, gi, D, pg.alpha);
+  }
   if (A.do_err) write_partial(A, w2);
 }
 
@@ -482,7 +499,10 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
         cmac(D, conv<R>(c.dir), in[static_cast<long long>(lk.nbr) * N2 + e]);
       }
     }
-    if (ok) err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, gi, D, pg.alpha);
+    if (ok) {
+      add_closure(d, gi, D);
+      err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, gi, D, pg.alpha);
+    }
     __syncthreads();
   }
   if (A.do_err) write_partial(A, err_acc);
@@ -576,6 +596,7 @@ __global__ void __launch_bounds__(kBigBS) hilb_big_kernel(const FormDev d, const
           }
           Pre pv;
           prefetch<F64, kMaxPre>(pv, A, cur, base + e, true);
+          add_closure(d, base + e, D);
           err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, base + e, D, pg.alpha);
         }
     }
@@ -860,6 +881,7 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
         }
         cmac(D, cf, xv);
       }
+      add_closure(d, base + x, D);
       err_acc += epilogue<F64, R, KP>(pv, A, cur, base + x, D, pg.alpha);
     }
     __syncthreads();
@@ -939,6 +961,24 @@ __global__ void __launch_bounds__(64) batch_kernel(const FormDev d, const StageA
     __syncthreads();
   }
   if (A.do_err) write_partial(A, err_acc);
+}
+
+// closure network input (Re, Im of the last-tier ADOs of the stage input)
+__global__ void closure_gather_kernel(const FormDev d, const StageArgs A, float* x,
+                                      int ado_stride) {
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const void* in = pick(A.in, pg.cur);
+  const long long n = static_cast<long long>(d.clos_n) * d.N2;
+  const long long base = static_cast<long long>(d.clos_lo) * d.N2;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const z128 v = ldz(in, A.in_lo != 0, base + i);
+    const long long r = i / d.N2;
+    const int e = static_cast<int>(i - r * d.N2);
+    x[r * ado_stride + e] = static_cast<float>(v.re);
+    x[r * ado_stride + d.N2 + e] = static_cast<float>(v.im);
+  }
 }
 
 // rho_S of every batch member, gathered into one block (member order)
@@ -1239,6 +1279,14 @@ void launch_err_finalize(const StageArgs& A, int nparts, hipStream_t stream) {
 
 void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t stream) {
   hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(64), 0, stream, c, t0, t_bound, dt);
+  HAMD_HIP(hipGetLastError());
+}
+
+void launch_closure_gather(const FormDev& d, const StageArgs& A, float* x, int ado_stride,
+                           hipStream_t stream) {
+  const long long n = static_cast<long long>(d.clos_n) * d.N2;
+  hipLaunchKernelGGL(closure_gather_kernel, dim3(cap_grid((n + 255) / 256)), dim3(256), 0, stream,
+                     d, A, x, ado_stride);
   HAMD_HIP(hipGetLastError());
 }
 

@@ -3,6 +3,8 @@
 // (heom_hilb.h:57-68, heom_liou.h:66-119, heom_ado.h:62-189) and the upload
 // of every table to HBM.
 #include "qme.h"
+#include "kernels.h"
+#include "mlp.h"
 
 #include <algorithm>
 #include <cmath>
@@ -190,7 +192,41 @@ void supop_axpy(SupOp& y, z128 alpha, const SupOp& x) {
 }
 }  // namespace
 
+void Qme::clear_closure() {
+  closure.reset();
+  dev.clos = nullptr;
+  dev.clos_lo = dev.clos_n = 0;
+}
+
+StageArgs Qme::diff_args(void* drho, const void* rho, z128 alpha, z128 beta) const {
+  const int l = lo() ? 1 : 0;
+  StageArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.mode = MODE_DIFF;
+  A.in[0] = A.in[1] = rho;
+  A.in_lo = l;
+  A.dst[0][0] = A.dst[0][1] = drho;
+  A.dst_lo[0] = l;
+  A.ynew_pre = A.yold_pre = -1;
+  if (beta.re != 0.0 || beta.im != 0.0) {
+    A.npre = 1;
+    A.pre[0][0] = A.pre[1][0] = drho;
+    A.pre_lo = l ? 1u : 0u;
+  }
+  A.a_re = alpha.re;
+  A.a_im = alpha.im;
+  A.b_re = beta.re;
+  A.b_im = beta.im;
+  return A;
+}
+
+int Qme::operate(int real_bytes, const StageArgs& A, hipStream_t st) {
+  if (closure) closure->forward(*this, A, st);
+  return launch_operator(dev, real_bytes, A, st);
+}
+
 void Qme::set_param() {
+  clear_closure();
   const int nn = static_cast<int>(noise.size());
   if (!H_set) throw HeomError{HEOM_ERR_ARG, "set_param: H not set"};
   if (nn == 0) throw HeomError{HEOM_ERR_ARG, "set_param: alloc_noises not called"};

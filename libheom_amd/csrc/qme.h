@@ -51,6 +51,8 @@ struct Hierarchy {
 // k >= the mode last incremented, while the total depth stays <= max_depth.
 Hierarchy build_hierarchy(int n_dim, int max_depth);
 
+struct Closure;
+
 class Qme {
  public:
   Qme(int kind, int dtype, int n_level, int max_depth, const heom_options& opt);
@@ -87,6 +89,14 @@ class Qme {
   long long ado_nnz = 0;
 
   void set_param();
+  // learned truncation closure (mlp.h), null when none is bound
+  std::unique_ptr<Closure> closure;
+  void clear_closure();
+  // arguments of one derivative pass drho = beta*drho - alpha*D(rho)
+  StageArgs diff_args(void* drho, const void* rho, z128 alpha, z128 beta) const;
+  DevBuf scratch;  // closure training rows: one state of this QME
+  // closure forward (when bound) followed by the operator pass
+  int operate(int real_bytes, const StageArgs& A, hipStream_t st);
   // pack already set-up Hilbert-form members (diagonal V, n_level <= 8) into
   // one batch whose state is the concatenation of the members' states
   void set_param_batch(const std::vector<Qme*>& members);

@@ -225,7 +225,7 @@ void Solver::build_stages(double atol, double rtol) {
 void Solver::launch_stage(int s, double dt) {
   StageArgs a = stages[s];
   a.a_re = dt;
-  int grid = launch_operator(q->dev, in_bytes, a, q->stream);
+  int grid = q->operate(in_bytes, a, q->stream);
   if (a.do_err) launch_err_finalize(a, grid, q->stream);
 }
 

@@ -167,6 +167,32 @@ def bench_sweep(H, L, steps, warmup, reps):
     }
 
 
+def bench_spin_boson(H):
+    """The CPU-runnable reference case: spin-boson N=2, 2 modes, depth 10 (66
+    ADOs), RK4 dt=0.01 x 2000 steps, rho_S every 10 steps.  Latency-bound (the
+    whole state is 4 KB): reported as time-to-solution with the stage kernels
+    replayed from HIP graphs and launched one by one."""
+    from libheom_amd import configs
+    m = configs.spin_boson()
+    eng = H.hip(0)
+    q = H.heom_hilb(m["max_depth"]).load(m, eng)
+    res = {}
+    for graphs in (True, False):
+        sv = H.qme_solver(eng, q, H.rk4())
+        sv.set_graphs(graphs)
+        sv.solve_rho_s(m["rho0"], m["t_list"][:3], {"dt": m["dt"]})  # warm-up
+        t0 = time.perf_counter()
+        sv.solve_rho_s(m["rho0"], m["t_list"], {"dt": m["dt"]})
+        res[graphs] = time.perf_counter() - t0
+        st = sv.stats()
+    return {"workload": "spin-boson N=2, 1 Drude + 1 Pade mode, depth 10 (66 ADOs), RK4 "
+                        "dt=0.01 x 2000 steps, rho_S every 10 steps",
+            "time_to_rho_s_ms_graphs": 1e3 * res[True],
+            "time_to_rho_s_ms_direct_launches": 1e3 * res[False],
+            "us_per_stage_graphs": 1e6 * res[True] / 8000,
+            "value_graphs": q.n_hrchy * 8000 / res[True]}
+
+
 def bench_dense64(H, L, reps):
     """Large-N config: N=64 GOE Hamiltonian, 4 projector baths x (1 Drude + 3
     Pade) = 16 modes, depth 3 (969 ADOs of 64x64), Dormand-Prince rtol=1e-6 to
@@ -347,6 +373,7 @@ def main():
         "cpu_baseline": None,
         "variants": {"complex64": variant32},
     }
+    out["variants"]["spin_boson"] = bench_spin_boson(H)
     if not args.no_dense:
         out["variants"]["dense64"] = bench_dense64(H, L, 5)
     if not args.no_sweep:

@@ -395,3 +395,7 @@ class qme_solver:
 
     def reset(self):
         _capi.check(_capi.load().heom_solver_reset(self._s))
+
+    def set_graphs(self, enable):
+        """Replay fixed stage sequences from HIP graphs (default) or launch directly."""
+        _capi.check(_capi.load().heom_solver_set_graphs(self._s, int(bool(enable))))

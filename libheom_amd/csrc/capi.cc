@@ -545,6 +545,14 @@ int heom_solver_reset(heom_solver_handle s) {
   });
 }
 
+int heom_solver_set_graphs(heom_solver_handle s, int enable) {
+  return guard([&] {
+    if (!s || !s->s) throw hamd::HeomError{HEOM_ERR_ARG, "null solver handle"};
+    s->s->use_graphs = enable != 0;
+    return HEOM_OK;
+  });
+}
+
 int heom_solver_time_stages(heom_solver_handle s, int reps, double* ms_per_stage, int* n_stage) {
   return guard([&] {
     if (!s || !s->s || !ms_per_stage) throw hamd::HeomError{HEOM_ERR_ARG, "NULL"};

@@ -229,23 +229,89 @@ void Solver::launch_stage(int s, double dt) {
   if (a.do_err) launch_err_finalize(a, grid, q->stream);
 }
 
+// Record the kernels `body` launches on the library stream into g (stream
+// capture; the stage launches enqueue kernels only, no memory operations).
+template <typename F>
+void Solver::capture(GraphExec& g, F&& body) {
+  g.reset();
+  hipStream_t st = q->stream;
+  HAMD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(st, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HAMD_HIP(hipStreamEndCapture(st, &graph));
+  const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HAMD_HIP(e);
+}
+
+void Solver::reset_graphs() {
+  g_rk4.clear();
+  g_dopri.clear();
+  g_rk4_dt = 0.0;
+}
+
+// Graph of n repetitions of `one` (cached by n; the cache holds a few sizes).
+GraphExec& Solver::graph_for(std::map<long long, std::unique_ptr<GraphExec>>& cache, long long n,
+                             const std::function<void()>& one) {
+  auto it = cache.find(n);
+  if (it != cache.end()) return *it->second;
+  if (cache.size() >= kMaxGraphs) cache.erase(cache.begin());
+  auto g = std::make_unique<GraphExec>();
+  capture(*g, [&] {
+    for (long long k = 0; k < n; ++k) one();
+  });
+  return *cache.emplace(n, std::move(g)).first->second;
+}
+
+// `count` repetitions of `one` as graph launches of power-of-two sizes
+void Solver::replay(std::map<long long, std::unique_ptr<GraphExec>>& cache, long long count,
+                    const std::function<void()>& one) {
+  while (count > 0) {
+    long long n = kMaxGraphSteps;
+    while (n > count) n >>= 1;
+    HAMD_HIP(hipGraphLaunch(graph_for(cache, n, one).exec, q->stream));
+    count -= n;
+  }
+}
+
 // fixed_step_size_solver::solve_1 (fixed_step_size_solver.h:29-52): steps of
-// dt, the last one shortened to end exactly at t1.
+// dt, the last one shortened to end exactly at t1.  The step sequence is
+// computed on the host with the reference's arithmetic; the steps of the full
+// dt are replayed from graphs of kGraphSteps steps and of one step.
 void Solver::rk4_interval(double t0, double t1, double dt) {
   double t = t0, dt_1 = dt;
-  bool brk = false;
+  long long n_full = 0;
   while (true) {
     if (t + dt_1 - t1 > -std::numeric_limits<double>::epsilon()) {
       dt_1 = t1 - t;
-      brk = true;
+      break;
     }
-    for (int s = 0; s < 4; ++s) launch_stage(s, dt_1);
-    stats.n_rhs += 4;
-    stats.n_attempts += 1;
-    stats.n_accepted += 1;
+    ++n_full;
     t += dt_1;
-    if (brk) break;
   }
+  long long done = 0;
+  if (use_graphs && n_full > 0) {
+    if (g_rk4_dt != dt) {
+      g_rk4.clear();
+      g_rk4_dt = dt;
+    }
+    replay(g_rk4, n_full, [&] {
+      for (int s = 0; s < 4; ++s) launch_stage(s, dt);
+    });
+    done = n_full;
+  }
+  for (; done < n_full; ++done)
+    for (int s = 0; s < 4; ++s) launch_stage(s, dt);
+  for (int s = 0; s < 4; ++s) launch_stage(s, dt_1);  // the last (shortened) step
+  stats.n_rhs += 4 * (n_full + 1);
+  stats.n_attempts += n_full + 1;
+  stats.n_accepted += n_full + 1;
 }
 
 // adaptive_step_size_solver::solve_1 (adaptive_step_size_solver.h:32-56) with
@@ -253,9 +319,15 @@ void Solver::rk4_interval(double t0, double t1, double dt) {
 int Solver::rkdp_interval(double t0, double t1, double dt_first, int& batch) {
   launch_ctrl_init(d_ctrl.as<Ctrl>(), t0, t1, dt_first, q->stream);
   long long a0 = h_ctrl->attempts;
+  const auto attempt = [&] {
+    for (int s = 0; s < 7; ++s) launch_stage(s, 0.0);
+  };
   while (true) {
-    for (int b = 0; b < batch; ++b)
-      for (int s = 0; s < 7; ++s) launch_stage(s, 0.0);
+    if (use_graphs) {
+      replay(g_dopri, batch, attempt);
+    } else {
+      for (int b = 0; b < batch; ++b) attempt();
+    }
     HAMD_HIP(hipMemcpyAsync(h_ctrl, d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, q->stream));
     HAMD_HIP(hipStreamSynchronize(q->stream));
     if (h_ctrl->done) break;
@@ -280,6 +352,7 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
   }
   HAMD_HIP(hipSetDevice(q->opt.device));
   build_stages(atol, rtol);
+  reset_graphs();  // stage arguments (tolerances, closure) may have changed
   const size_t bytes = sizeof(z128) * ms;
   hipStream_t st = q->stream;
   // the caller's state enters the FP64 state slot

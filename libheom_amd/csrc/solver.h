@@ -2,6 +2,8 @@
 #pragma once
 
 #include <functional>
+#include <map>
+#include <memory>
 #include <vector>
 
 #include "qme.h"
@@ -28,6 +30,19 @@ struct StageSpec {
   void combo(const std::vector<std::pair<int, double>>& terms, double ckv, int d, int d2 = -1);
 };
 
+// An instantiated HIP graph owned by the solver.
+struct GraphExec {
+  hipGraphExec_t exec = nullptr;
+  GraphExec() = default;
+  GraphExec(const GraphExec&) = delete;
+  GraphExec& operator=(const GraphExec&) = delete;
+  ~GraphExec() { reset(); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+};
+
 class Solver {
  public:
   Solver(Qme* q, int kind);
@@ -39,6 +54,9 @@ class Solver {
   long long ms;    // main_size
   double dt_save = -1.0;
   heom_stats stats{};
+  // replay fixed stage sequences from HIP graphs (RK4 steps of the full dt,
+  // Dormand-Prince attempts); 0 launches every kernel individually
+  bool use_graphs = true;
 
   // Propagate the FP64 device state rho_dev over t_list.  on_output(i, cur_ptr)
   // is called at every t_i with a device pointer to the FP64 state.
@@ -66,6 +84,19 @@ class Solver {
   StageArgs resolve(const StageSpec& s) const;
   std::vector<StageSpec> specs;
   void launch_stage(int s, double dt);
+  // graphs of 2^j full RK4 steps / Dormand-Prince attempts (j <= 8), by 2^j;
+  // a count is replayed as its binary decomposition
+  static constexpr int kMaxGraphSteps = 256;
+  static constexpr size_t kMaxGraphs = 9;
+  std::map<long long, std::unique_ptr<GraphExec>> g_rk4, g_dopri;
+  double g_rk4_dt = 0.0;
+  GraphExec& graph_for(std::map<long long, std::unique_ptr<GraphExec>>& cache, long long n,
+                       const std::function<void()>& one);
+  void replay(std::map<long long, std::unique_ptr<GraphExec>>& cache, long long count,
+              const std::function<void()>& one);
+  template <typename F>
+  void capture(GraphExec& g, F&& body);
+  void reset_graphs();
   void rk4_interval(double t0, double t1, double dt);
   int rkdp_interval(double t0, double t1, double dt_first, int& batch);
 };

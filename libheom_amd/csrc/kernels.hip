@@ -610,9 +610,8 @@ __global__ void __launch_bounds__(kBigBS) hilb_big_kernel(const FormDev d, const
 // matrix cores (heom_hilb.h:108-125 computes it with two gemm calls per ADO).
 // One ADO per 256-thread workgroup.  rho_n is staged in LDS as separate real
 // and imaginary planes with row stride N+1, so that the 16 rows of an MFMA
-// fragment fall in distinct banks.  H never changes: each wave keeps its
-// 16-row band of H (the A operand of H*rho) in registers for the whole launch
-// and reads the B operand of rho*H from L2.  Wave w owns NB*NB/4 of the
+// fragment fall in distinct banks.  H (shared by every ADO) is read from L2
+// as MFMA operands.  Wave w owns NB*NB/4 of the
 // (N/16)^2 output tiles, all in one 16-row band, and accumulates H*rho and
 // -rho*H into the same accumulators: 4 real products per k step when H is
 // real, 8 otherwise.  The accumulators then feed the element-wise part of
@@ -692,9 +691,7 @@ struct MfmaCfg {
 // H*rho - rho*H for the wave's tiles, operands in the state precision R
 template <int N, typename R, bool REALH>
 __device__ __forceinline__ void mfma_commutator(const FormDev& d, const R* s_re, const R* s_im,
-                                                const R (&ah_re)[MfmaCfg<N>::KS],
-                                                const R (&ah_im)[MfmaCfg<N>::KS], int band,
-                                                int t0, int lr, int lk,
+                                                int band, int t0, int lr, int lk,
                                                 typename Mfma<R>::acc (&cre)[MfmaCfg<N>::TPW],
                                                 typename Mfma<R>::acc (&cim)[MfmaCfg<N>::TPW]) {
   using M = Mfma<R>;
@@ -706,7 +703,7 @@ __device__ __forceinline__ void mfma_commutator(const FormDev& d, const R* s_re,
   for (int ks = 0; ks < Cfg::KS; ++ks) {
     const int kk = 4 * ks + lk;
     const R arr = -s_re[row * Cfg::LD + kk], ari = -s_im[row * Cfg::LD + kk];
-    const R ahr = ah_re[ks], ahi = ah_im[ks];
+    const R ahr = Hr[row * N + kk], ahi = REALH ? R(0) : Hi[row * N + kk];
 #pragma unroll
     for (int t = 0; t < Cfg::TPW; ++t) {
       const int col = ((t0 + t) % Cfg::NB) * 16 + lr;
@@ -808,24 +805,22 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
   const int lr = lane & 15, lk = lane >> 4;
   const int t0 = wave * TPW;
   const int band = t0 / NB;
-  // the wave's band of H as MFMA A operands, kept for the whole launch
-  R ah_re[Cfg::KS], ah_im[Cfg::KS];
-  if (!BF16X3) {
-#pragma unroll
-    for (int ks = 0; ks < Cfg::KS; ++ks) {
-      const int e = (band * 16 + lr) * N + 4 * ks + lk;
-      ah_re[ks] = M::re(d)[e];
-      ah_im[ks] = REALH ? R(0) : M::im(d)[e];
-    }
-  }
   double err_acc = 0.0;
   for (int g = blockIdx.x; g < d.n_hrchy; g += gridDim.x) {
     const long long base = static_cast<long long>(g) * N2;
-    for (int x = threadIdx.x; x < N2; x += kMfmaBS) {
-      const C v = in[base + x];
-      const int i = x / N, j = x % N;
-      s_re[i * LD + j] = v.re;
-      s_im[i * LD + j] = v.im;
+    {
+      // all loads of rho_n in flight before the LDS stores
+      constexpr int PER = N2 / kMfmaBS;
+      C v[PER];
+#pragma unroll
+      for (int c = 0; c < PER; ++c) v[c] = in[base + threadIdx.x + c * kMfmaBS];
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const int x = threadIdx.x + c * kMfmaBS;
+        const int i = x / N, j = x % N;
+        s_re[i * LD + j] = v[c].re;
+        s_im[i * LD + j] = v[c].im;
+      }
     }
     __syncthreads();
     typename M::acc cre[TPW], cim[TPW];
@@ -837,7 +832,7 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
     if constexpr (BF16X3)
       mfma_commutator_bf16x3<N, REALH>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
     else
-      mfma_commutator<N, R, REALH>(d, s_re, s_im, ah_re, ah_im, band, t0, lr, lk, cre, cim);
+      mfma_commutator<N, R, REALH>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
     // every wave has read rho_n as MFMA operands: each lane now replaces its
     // own elements by i[H, rho] + (Gamma_n + delta term) rho (one owner each)
     __syncthreads();
@@ -857,32 +852,58 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
     }
     __syncthreads();
     // neighbour couplings and the Runge-Kutta epilogue, element-parallel and
-    // coalesced
+    // coalesced, CH elements per thread and pass: every load of a pass is
+    // issued before its stores (the stores may alias later loads)
+    constexpr int CH = KP <= 4 ? 2 : 1;
+    static_assert(N2 % (kMfmaBS * CH) == 0, "element chunks must tile the ADO");
     const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
-    for (int x = threadIdx.x; x < N2; x += kMfmaBS) {
-      const int i = x / N, j = x % N;
-      C D{s_re[i * LD + j], s_im[i * LD + j]};
-      PreT<KP> pv;
-      prefetch<F64, KP>(pv, A, cur, base + x, true);
+    for (int x0 = threadIdx.x; x0 < N2; x0 += kMfmaBS * CH) {
+      C Dc[CH];
+      PreT<KP> pv[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int x = x0 + c * kMfmaBS;
+        const int i = x / N, j = x % N;
+        Dc[c] = C{s_re[i * LD + j], s_im[i * LD + j]};
+        prefetch<F64, KP>(pv[c], A, cur, base + x, true);
+      }
       for (int l = l0; l < l1; ++l) {
         const LinkFlat q = ld_flat(flat + l);
-        const C xv = in[static_cast<long long>(q.nbr) * N2 + x];
-        const long long tb = static_cast<long long>(q.u) * N2 + x;
-        C cf;
-        if (d.real_v) {
-          const z128 p = d.cmac_tab[tb];
-          cf.re = static_cast<R>(p.re) * static_cast<R>(q.iphi.re) +
-                  static_cast<R>(p.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
-          cf.im = static_cast<R>(p.re) * static_cast<R>(q.iphi.im) +
-                  static_cast<R>(p.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
-        } else {
-          cf = conv<R>(d.cmac_tab[2 * tb]) * conv<R>(q.iphi) +
-               conv<R>(d.cmac_tab[2 * tb + 1]) * conv<R>(q.psi) + conv<R>(q.dir);
+        C xv[CH];
+        z128 p0[CH], p1[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const int x = x0 + c * kMfmaBS;
+          xv[c] = in[static_cast<long long>(q.nbr) * N2 + x];
+          const long long tb = static_cast<long long>(q.u) * N2 + x;
+          if (d.real_v) {
+            p0[c] = d.cmac_tab[tb];
+          } else {
+            p0[c] = d.cmac_tab[2 * tb];
+            p1[c] = d.cmac_tab[2 * tb + 1];
+          }
         }
-        cmac(D, cf, xv);
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          C cf;
+          if (d.real_v) {
+            cf.re = static_cast<R>(p0[c].re) * static_cast<R>(q.iphi.re) +
+                    static_cast<R>(p0[c].im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
+            cf.im = static_cast<R>(p0[c].re) * static_cast<R>(q.iphi.im) +
+                    static_cast<R>(p0[c].im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
+          } else {
+            cf = conv<R>(p0[c]) * conv<R>(q.iphi) + conv<R>(p1[c]) * conv<R>(q.psi) +
+                 conv<R>(q.dir);
+          }
+          cmac(Dc[c], cf, xv[c]);
+        }
       }
-      add_closure(d, base + x, D);
-      err_acc += epilogue<F64, R, KP>(pv, A, cur, base + x, D, pg.alpha);
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int x = x0 + c * kMfmaBS;
+        add_closure(d, base + x, Dc[c]);
+        err_acc += epilogue<F64, R, KP>(pv[c], A, cur, base + x, Dc[c], pg.alpha);
+      }
     }
     __syncthreads();
   }

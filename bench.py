@@ -266,6 +266,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--no-spin", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every stage kernel directly (profiler runs)")
     ap.add_argument("--reps", type=int, default=20)
@@ -399,7 +400,8 @@ def main():
         "cpu_baseline": None,
         "variants": {"complex64": variant32, "host_buffers": host},
     }
-    out["variants"]["spin_boson"] = bench_spin_boson(H)
+    if not args.no_spin:
+        out["variants"]["spin_boson"] = bench_spin_boson(H)
     if not args.no_dense:
         out["variants"]["dense64"] = bench_dense64(H, L, 5)
     if not args.no_sweep:

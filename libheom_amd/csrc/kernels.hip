@@ -372,7 +372,7 @@ struct HilbCfg {
   static constexpr int APB = BS / N2;  // ADOs per workgroup
 };
 
-template <int N, typename R, bool DIAGV, bool F64>
+template <int N, typename R, bool DIAGV, bool F64, int KP>
 __global__ void __launch_bounds__(HilbCfg<N>::BS)
     hilb_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
@@ -408,8 +408,8 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
     const bool ok = lane_ok && n < d.n_hrchy;
     const long long gi = static_cast<long long>(n) * N2 + e;
     const C r = ok ? in[gi] : C{0, 0};
-    Pre pv;
-    prefetch<F64, kMaxPre>(pv, A, cur, gi, ok);
+    PreT<KP> pv;
+    prefetch<F64, KP>(pv, A, cur, gi, ok);
     const int l0 = ok ? d.link_ptr[n] : 0;
     const int l1 = ok ? d.link_ptr[n + 1] : 0;
     if (lane_ok) s_rho[tid] = r;
@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
     }
     if (ok) {
       add_closure(d, gi, D);
-      err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, gi, D, pg.alpha);
+      err_acc += epilogue<F64, R, KP>(pv, A, cur, gi, D, pg.alpha);
     }
     __syncthreads();
   }
@@ -517,7 +517,7 @@ __global__ void __launch_bounds__(HilbCfg<N>::BS)
 constexpr int kBigBS = 256;
 constexpr int kBigTile = 4;
 
-template <typename R, bool HLDS, bool F64>
+template <typename R, bool HLDS, bool F64, int KP>
 __global__ void __launch_bounds__(kBigBS) hilb_big_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   extern __shared__ __align__(16) unsigned char s_dyn[];
@@ -594,10 +594,10 @@ __global__ void __launch_bounds__(kBigBS) hilb_big_kernel(const FormDev d, const
             }
             cmac(D, cf, x);
           }
-          Pre pv;
-          prefetch<F64, kMaxPre>(pv, A, cur, base + e, true);
+          PreT<KP> pv;
+          prefetch<F64, KP>(pv, A, cur, base + e, true);
           add_closure(d, base + e, D);
-          err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, base + e, D, pg.alpha);
+          err_acc += epilogue<F64, R, KP>(pv, A, cur, base + e, D, pg.alpha);
         }
     }
     __syncthreads();
@@ -915,7 +915,64 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
 // launch advances every member; a 64-lane wavefront takes one work item
 // (64/N^2 ADOs of one member)
 // ---------------------------------------------------------------------------
-template <typename R, bool F64>
+// One work item of the batch kernel: apw = 64/N^2 ADOs of one member.
+template <typename R, bool F64, int KP>
+__device__ __forceinline__ void batch_item(const FormDev& d, const StageArgs& A, const cx<R>* in,
+                                           const BatchInst& I, int ado0, int cur, z128 alpha,
+                                           cx<R>* s_rho, cx<R>* s_H, double& err_acc) {
+  using C = cx<R>;
+  const int lane = threadIdx.x;
+  const int N = I.N, N2 = I.N2, apw = I.apw;
+  const int la = lane / N2;
+  const int e = lane - la * N2;
+  const int i = e / N, j = e - (e / N) * N;
+  const int ado = ado0 + la;
+  const bool ok = la < apw && ado < I.n_hrchy;
+  const long long gi = I.elem_base + static_cast<long long>(ok ? ado : ado0) * N2 + (ok ? e : 0);
+  const C r = in[gi];
+  PreT<KP> pv;
+  prefetch<F64, KP>(pv, A, cur, gi, ok);
+  s_H[lane] = lane < N2 ? conv<R>(d.H[I.h_off + lane]) : C{0, 0};
+  s_rho[lane] = ok ? r : C{0, 0};
+  __syncthreads();
+  C acc{0, 0};
+  if (ok) {
+    const C* b = s_rho + la * N2;
+    for (int k = 0; k < N; ++k) {
+      cmac(acc, s_H[i * N + k], b[k * N + j]);
+      const C t = b[i * N + k];
+      cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
+    }
+  }
+  C D = times_i(acc);
+  if (ok) {
+    cmac(D, conv<R>(d.gam[I.ado_base + ado]) + conv<R>(d.dterm[I.dterm_off + e]), r);
+    const int l0 = d.link_ptr[I.ado_base + ado], l1 = d.link_ptr[I.ado_base + ado + 1];
+    for (int l = l0; l < l1; ++l) {
+      const LinkFlat q = d.flat[l];
+      const C x = in[I.elem_base + static_cast<long long>(q.nbr) * N2 + e];
+      const int tb = q.u * N2 + e;
+      C cf;
+      if (I.real_v) {
+        const z128 p = d.cmac_tab[I.tab_off + tb];
+        cf.re = static_cast<R>(p.re) * static_cast<R>(q.iphi.re) +
+                static_cast<R>(p.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
+        cf.im = static_cast<R>(p.re) * static_cast<R>(q.iphi.im) +
+                static_cast<R>(p.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
+      } else {
+        cf = conv<R>(d.cmac_tab[I.tab_off + 2 * tb]) * conv<R>(q.iphi) +
+             conv<R>(d.cmac_tab[I.tab_off + 2 * tb + 1]) * conv<R>(q.psi) + conv<R>(q.dir);
+      }
+      cmac(D, cf, x);
+    }
+    err_acc += epilogue<F64, R, KP>(pv, A, cur, gi, D, alpha);
+  }
+  __syncthreads();
+}
+
+// KP = 4 for the stages with at most four epilogue operands (every RK4 stage):
+// the smaller operand set keeps 7 waves per SIMD (FP32) instead of 4-5
+template <typename R, bool F64, int KP>
 __global__ void __launch_bounds__(64) batch_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   __shared__ C s_rho[64];
@@ -924,62 +981,23 @@ __global__ void __launch_bounds__(64) batch_kernel(const FormDev d, const StageA
   if (pg.skip) return;
   const int cur = pg.cur;
   const C* in = static_cast<const C*>(pick(A.in, cur));
-  const int lane = threadIdx.x;
   const auto items = cst(d.items);
   const auto insts = cst(d.inst);
   double err_acc = 0.0;
   for (int it = blockIdx.x; it < d.n_items; it += gridDim.x) {
-    const int inst = items[it].inst, ado0 = items[it].ado;
-    const int N = insts[inst].N, N2 = insts[inst].N2, apw = insts[inst].apw;
-    const int nh = insts[inst].n_hrchy, abase = insts[inst].ado_base;
-    const int hoff = insts[inst].h_off, toff = insts[inst].tab_off;
-    const int doff = insts[inst].dterm_off, realv = insts[inst].real_v;
-    const long long ebase = insts[inst].elem_base;
-    const int la = lane / N2;
-    const int e = lane - la * N2;
-    const int i = e / N, j = e - (e / N) * N;
-    const int ado = ado0 + la;
-    const bool ok = la < apw && ado < nh;
-    const long long gi = ebase + static_cast<long long>(ok ? ado : ado0) * N2 + (ok ? e : 0);
-    const C r = in[gi];
-    Pre pv;
-    prefetch<F64, kMaxPre>(pv, A, cur, gi, ok);
-    s_H[lane] = lane < N2 ? conv<R>(d.H[hoff + lane]) : C{0, 0};
-    s_rho[lane] = ok ? r : C{0, 0};
-    __syncthreads();
-    C acc{0, 0};
-    if (ok) {
-      const C* b = s_rho + la * N2;
-      for (int k = 0; k < N; ++k) {
-        cmac(acc, s_H[i * N + k], b[k * N + j]);
-        const C t = b[i * N + k];
-        cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
-      }
-    }
-    C D = times_i(acc);
-    if (ok) {
-      cmac(D, conv<R>(d.gam[abase + ado]) + conv<R>(d.dterm[doff + e]), r);
-      const int l0 = d.link_ptr[abase + ado], l1 = d.link_ptr[abase + ado + 1];
-      for (int l = l0; l < l1; ++l) {
-        const LinkFlat q = d.flat[l];
-        const C x = in[ebase + static_cast<long long>(q.nbr) * N2 + e];
-        const int tb = q.u * N2 + e;
-        C cf;
-        if (realv) {
-          const z128 p = d.cmac_tab[toff + tb];
-          cf.re = static_cast<R>(p.re) * static_cast<R>(q.iphi.re) +
-                  static_cast<R>(p.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
-          cf.im = static_cast<R>(p.re) * static_cast<R>(q.iphi.im) +
-                  static_cast<R>(p.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
-        } else {
-          cf = conv<R>(d.cmac_tab[toff + 2 * tb]) * conv<R>(q.iphi) +
-               conv<R>(d.cmac_tab[toff + 2 * tb + 1]) * conv<R>(q.psi) + conv<R>(q.dir);
-        }
-        cmac(D, cf, x);
-      }
-      err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, gi, D, pg.alpha);
-    }
-    __syncthreads();
+    const int m = items[it].inst, ado0 = items[it].ado;
+    BatchInst I;
+    I.N = insts[m].N;
+    I.N2 = insts[m].N2;
+    I.n_hrchy = insts[m].n_hrchy;
+    I.apw = insts[m].apw;
+    I.ado_base = insts[m].ado_base;
+    I.h_off = insts[m].h_off;
+    I.tab_off = insts[m].tab_off;
+    I.dterm_off = insts[m].dterm_off;
+    I.real_v = insts[m].real_v;
+    I.elem_base = insts[m].elem_base;
+    batch_item<R, F64, KP>(d, A, in, I, ado0, cur, pg.alpha, s_rho, s_H, err_acc);
   }
   if (A.do_err) write_partial(A, err_acc);
 }
@@ -1011,7 +1029,7 @@ __global__ void gather_rho_s_kernel(z128* dst, const z128* src, const BatchInst*
 // ---------------------------------------------------------------------------
 // heom_liou: per-ADO superoperator form (heom_liou.h:151-255), N <= 16
 // ---------------------------------------------------------------------------
-template <typename R, bool F64>
+template <typename R, bool F64, int KP>
 __global__ void __launch_bounds__(256) liou_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   __shared__ C s_rho[256];
@@ -1035,8 +1053,8 @@ __global__ void __launch_bounds__(256) liou_kernel(const FormDev d, const StageA
     const bool ok = lane_ok && n < d.n_hrchy;
     const long long gi = static_cast<long long>(n) * N2 + e;
     const C r = ok ? in[gi] : C{0, 0};
-    Pre pv;
-    prefetch<F64, kMaxPre>(pv, A, cur, gi, ok);
+    PreT<KP> pv;
+    prefetch<F64, KP>(pv, A, cur, gi, ok);
     if (lane_ok) s_rho[tid] = r;
     __syncthreads();
     C D{0, 0};
@@ -1080,7 +1098,7 @@ __global__ void __launch_bounds__(256) liou_kernel(const FormDev d, const StageA
       const Coef c = d.coef[lk.cid];
       cmac(D, conv<R>(c.dir), in[static_cast<long long>(lk.nbr) * N2 + e]);
     }
-    if (ok) err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, gi, D, pg.alpha);
+    if (ok) err_acc += epilogue<F64, R, KP>(pv, A, cur, gi, D, pg.alpha);
     __syncthreads();
   }
   if (A.do_err) write_partial(A, err_acc);
@@ -1089,7 +1107,7 @@ __global__ void __launch_bounds__(256) liou_kernel(const FormDev d, const StageA
 // ---------------------------------------------------------------------------
 // heom_ado: one global CSR matrix-vector product (heom_ado.h:191-202)
 // ---------------------------------------------------------------------------
-template <typename R, bool F64>
+template <typename R, bool F64, int KP>
 __global__ void __launch_bounds__(256) ado_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   const Prologue pg = prologue(A);
@@ -1100,12 +1118,12 @@ __global__ void __launch_bounds__(256) ado_kernel(const FormDev d, const StageAr
   const long long rows = d.main_size;
   for (long long r = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; r < rows;
        r += static_cast<long long>(gridDim.x) * blockDim.x) {
-    Pre pv;
-    prefetch<F64, kMaxPre>(pv, A, cur, r, true);
+    PreT<KP> pv;
+    prefetch<F64, KP>(pv, A, cur, r, true);
     C D{0, 0};
     for (long long p = d.R_ptr[r]; p < d.R_ptr[r + 1]; ++p)
       cmac(D, conv<R>(d.R_val[p]), in[d.R_col[p]]);
-    err_acc += epilogue<F64, R, kMaxPre>(pv, A, cur, r, D, pg.alpha);
+    err_acc += epilogue<F64, R, KP>(pv, A, cur, r, D, pg.alpha);
   }
   if (A.do_err) write_partial(A, err_acc);
 }
@@ -1167,13 +1185,22 @@ int launch_hilb_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
   }
   constexpr int APB = HilbCfg<N>::APB, BS = HilbCfg<N>::BS;
   const int grid = cap_grid((d.n_hrchy + APB - 1) / APB);
+  const bool small = A.npre <= 4 && !A.do_err;
   if (d.diag_v) {
     const size_t lds = sizeof(cx<R>) * (d.n_noise + 1) * N;
     if (lds > 48 * 1024)
       throw HeomError{HEOM_ERR_UNSUPPORTED, "heom_hilb: too many noises for the LDS tables"};
-    hipLaunchKernelGGL((hilb_kernel<N, R, true, F64>), dim3(grid), dim3(BS), lds, st, d, A);
+    if (small)
+      hipLaunchKernelGGL((hilb_kernel<N, R, true, F64, 4>), dim3(grid), dim3(BS), lds, st, d, A);
+    else
+      hipLaunchKernelGGL((hilb_kernel<N, R, true, F64, kMaxPre>), dim3(grid), dim3(BS), lds, st,
+                         d, A);
   } else {
-    hipLaunchKernelGGL((hilb_kernel<N, R, false, F64>), dim3(grid), dim3(BS), 0, st, d, A);
+    if (small)
+      hipLaunchKernelGGL((hilb_kernel<N, R, false, F64, 4>), dim3(grid), dim3(BS), 0, st, d, A);
+    else
+      hipLaunchKernelGGL((hilb_kernel<N, R, false, F64, kMaxPre>), dim3(grid), dim3(BS), 0, st, d,
+                         A);
   }
   return grid;
 }
@@ -1231,11 +1258,19 @@ int launch_hilb_big(const FormDev& d, const StageArgs& A, hipStream_t st) {
   const size_t one = sizeof(cx<R>) * static_cast<size_t>(d.N2);
   const int grid = cap_grid(d.n_hrchy);
   if (2 * one <= 64 * 1024) {
-    hipLaunchKernelGGL((hilb_big_kernel<R, true, F64>), dim3(grid), dim3(kBigBS), 2 * one, st, d,
-                       A);
+    if (A.npre <= 4 && !A.do_err)
+      hipLaunchKernelGGL((hilb_big_kernel<R, true, F64, 4>), dim3(grid), dim3(kBigBS), 2 * one,
+                         st, d, A);
+    else
+      hipLaunchKernelGGL((hilb_big_kernel<R, true, F64, kMaxPre>), dim3(grid), dim3(kBigBS),
+                         2 * one, st, d, A);
   } else if (one <= 64 * 1024) {
-    hipLaunchKernelGGL((hilb_big_kernel<R, false, F64>), dim3(grid), dim3(kBigBS), one, st, d,
-                       A);
+    if (A.npre <= 4 && !A.do_err)
+      hipLaunchKernelGGL((hilb_big_kernel<R, false, F64, 4>), dim3(grid), dim3(kBigBS), one, st,
+                         d, A);
+    else
+      hipLaunchKernelGGL((hilb_big_kernel<R, false, F64, kMaxPre>), dim3(grid), dim3(kBigBS), one,
+                         st, d, A);
   } else {
     throw HeomError{HEOM_ERR_UNSUPPORTED, "heom_hilb: one ADO block exceeds 64 KB of LDS"};
   }
@@ -1262,17 +1297,26 @@ int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
   if (d.kind == 0) return launch_hilb<R, F64>(d, A, st);
   if (d.kind == kBatch) {
     const int grid = d.batch_grid > 0 ? d.batch_grid : 1;
-    hipLaunchKernelGGL((batch_kernel<R, F64>), dim3(grid), dim3(64), 0, st, d, A);
+    if (A.npre <= 4 && !A.do_err)
+      hipLaunchKernelGGL((batch_kernel<R, F64, 4>), dim3(grid), dim3(64), 0, st, d, A);
+    else
+      hipLaunchKernelGGL((batch_kernel<R, F64, kMaxPre>), dim3(grid), dim3(64), 0, st, d, A);
     return grid;
   }
   if (d.kind == 1) {
     const int APB = 256 / d.N2;
     const int grid = cap_grid((d.n_hrchy + APB - 1) / APB);
-    hipLaunchKernelGGL((liou_kernel<R, F64>), dim3(grid), dim3(256), 0, st, d, A);
+    if (A.npre <= 4 && !A.do_err)
+      hipLaunchKernelGGL((liou_kernel<R, F64, 4>), dim3(grid), dim3(256), 0, st, d, A);
+    else
+      hipLaunchKernelGGL((liou_kernel<R, F64, kMaxPre>), dim3(grid), dim3(256), 0, st, d, A);
     return grid;
   }
   const int grid = cap_grid((d.main_size + 255) / 256);
-  hipLaunchKernelGGL((ado_kernel<R, F64>), dim3(grid), dim3(256), 0, st, d, A);
+  if (A.npre <= 4 && !A.do_err)
+    hipLaunchKernelGGL((ado_kernel<R, F64, 4>), dim3(grid), dim3(256), 0, st, d, A);
+  else
+    hipLaunchKernelGGL((ado_kernel<R, F64, kMaxPre>), dim3(grid), dim3(256), 0, st, d, A);
   return grid;
 }
 

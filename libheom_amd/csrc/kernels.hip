@@ -938,7 +938,11 @@ __device__ __forceinline__ void batch_item(const FormDev& d, const StageArgs& A,
   C acc{0, 0};
   if (ok) {
     const C* b = s_rho + la * N2;
-    for (int k = 0; k < N; ++k) {
+    // members have N <= 8: unrolled to 8 with an exit (measured faster than the
+    // rolled loop in FP64)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k >= N) break;
       cmac(acc, s_H[i * N + k], b[k * N + j]);
       const C t = b[i * N + k];
       cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
@@ -1165,9 +1169,13 @@ int launch_hilb_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
     const int grid = d.n_hrchy;
     // four prefetched operands cover every stage except the last Dormand-Prince one
     const bool small = A.npre <= 4 && !A.do_err;
+    const bool mid = A.npre <= 6 && !A.do_err;  // Dormand-Prince stages 5 and 6
     if (d.real_v) {
       if (small)
         hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, 4>), dim3(grid), dim3(BS), 0, st,
+                           d, A);
+      else if (mid)
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, 6>), dim3(grid), dim3(BS), 0, st,
                            d, A);
       else
         hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, kMaxPre>), dim3(grid), dim3(BS), 0,

@@ -97,6 +97,16 @@ int heom_set_noise(heom_handle h, int u, const double* V, int len_gamma,
                    const double* S, const double* A, double s_delta_re,
                    double s_delta_im);
 
+/* hrchy_filter of alloc_hrchy_space (hrchy_space.h:29-36): when set, an ADO
+ * index is enumerated only if filter(index, n_dim, depth, user) returns
+ * non-zero, tested where the breadth-first enumeration pushes it
+ * (hrchy_space.cc:183-187), so indices reached only through a rejected one are
+ * pruned too and their links become void.  The reference's heom classes pass
+ * none (heom.h:77-91); here any QME can take one.  NULL removes it; takes
+ * effect at the next heom_set_param. */
+typedef int (*heom_hrchy_filter)(const int* index, int n_dim, int depth, void* user);
+int heom_set_hrchy_filter(heom_handle h, heom_hrchy_filter filter, void* user);
+
 /* set_param (heom.h:57-139 + heom_hilb.h:57-68 / heom_liou.h:66-119 /
  * heom_ado.h:62-189): builds the hierarchy, neighbour links and coefficient
  * tables and uploads them to HBM. */

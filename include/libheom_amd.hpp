@@ -171,6 +171,7 @@ class qme_base {
       check(heom_set_noise(h_, u, Vd.data(), len, g.data(), p0.data(), sg.data(), Sd.data(),
                            Ad.data(), std::real(s_delta[u]), std::imag(s_delta[u])));
     }
+    if (hrchy_filter) check(heom_set_hrchy_filter(h_, &filter_trampoline, this));
     check(heom_set_param(h_));
     check(heom_get_n_hrchy(h_, &n_hrchy_));
   }
@@ -186,12 +187,19 @@ class qme_base {
 
   heom_handle handle() const { return h_; }
 
+  // hrchy_filter of alloc_hrchy_space (hrchy_space.h:29-36); empty: none
+  std::function<bool(std::vector<int>, int)> hrchy_filter;
+
   // reference quirk switches (parity defaults)
   int parity_q1 = 1;
   int sqrt_normalization = 0;
   int matrix_cores = 1;  // MFMA commutator where provided (n_level 32 / 64, diagonal V)
 
  protected:
+  static int filter_trampoline(const int* index, int n_dim, int depth, void* user) {
+    const auto* self = static_cast<const qme_base*>(user);
+    return self->hrchy_filter(std::vector<int>(index, index + n_dim), depth) ? 1 : 0;
+  }
   int kind_ = HEOM_HILB;
   int max_depth_ = 0;
   int n_hrchy_ = 0;

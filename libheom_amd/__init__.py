@@ -78,6 +78,9 @@ class _heom:
         self.n_level_2 = 0
         self.H = None
         self.n_noise = 0
+        # hrchy_filter(index: ndarray of n_dim ints, depth: int) -> bool, applied
+        # where the breadth-first enumeration adds an index (hrchy_space.cc:183-187)
+        self.hrchy_filter = None
         self._h = None
         self._ready = False
 
@@ -137,7 +140,23 @@ class _heom:
             sd = complex(self.s_delta[u])
             _capi.check(L.heom_set_noise(h, u, _ptr(V), ln, _ptr(g), _ptr(p0), _ptr(sg),
                                          _ptr(S), _ptr(A), sd.real, sd.imag))
+        errors = []
+        if self.hrchy_filter is not None:
+            user_filter = self.hrchy_filter
+
+            def _filter(idx, n_dim, depth, user):
+                try:
+                    return 1 if user_filter(np.ctypeslib.as_array(idx, shape=(n_dim,)).copy(),
+                                            depth) else 0
+                except Exception as e:  # never unwind through the C frame
+                    errors.append(e)
+                    return 0
+
+            self._filter_cb = _capi.FILTER(_filter)
+            _capi.check(L.heom_set_hrchy_filter(h, self._filter_cb, None))
         _capi.check(L.heom_set_param(h))
+        if errors:
+            raise errors[0]
         n = ctypes.c_int()
         _capi.check(L.heom_get_n_hrchy(h, ctypes.byref(n)))
         self.n_hrchy = n.value

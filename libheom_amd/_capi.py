@@ -26,7 +26,7 @@ HEOM_CLOSURE_PER_ADO, HEOM_CLOSURE_GLOBAL = 0, 1
 # every symbol declared in include/libheom_amd.h
 EXPORTS = [
     "heom_last_error", "heom_version", "heom_default_options", "heom_create", "heom_destroy",
-    "heom_set_H", "heom_alloc_noises", "heom_set_noise", "heom_set_param", "heom_get_n_hrchy",
+    "heom_set_H", "heom_alloc_noises", "heom_set_noise", "heom_set_hrchy_filter", "heom_set_param", "heom_get_n_hrchy",
     "heom_get_n_dim", "heom_main_size", "heom_get_hrchy", "heom_calc_diff",
     "heom_calc_diff_host", "heom_batch_create", "heom_batch_layout", "heom_solver_create", "heom_solver_destroy", "heom_solve",
     "heom_solve_dev", "heom_solver_stats", "heom_solver_reset", "heom_solver_set_graphs", "heom_solver_time_stages",
@@ -63,6 +63,10 @@ class heom_stats(ctypes.Structure):
 CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 
+# int filter(const int* index, int n_dim, int depth, void* user)
+FILTER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                          ctypes.c_int, ctypes.c_void_p)
+
 _lib = None
 
 
@@ -86,6 +90,7 @@ def load():
         "heom_set_H": (ip, [vp, vp]),
         "heom_alloc_noises": (ip, [vp, ip]),
         "heom_set_noise": (ip, [vp, ip, vp, ip, vp, vp, vp, vp, vp, dp, dp]),
+        "heom_set_hrchy_filter": (ip, [vp, FILTER, vp]),
         "heom_set_param": (ip, [vp]),
         "heom_get_n_hrchy": (ip, [vp, P(ip)]),
         "heom_get_n_dim": (ip, [vp, P(ip)]),

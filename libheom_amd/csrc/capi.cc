@@ -181,6 +181,16 @@ int heom_set_noise(heom_handle h, int u, const double* V, int len_gamma, const d
   });
 }
 
+int heom_set_hrchy_filter(heom_handle h, heom_hrchy_filter filter, void* user) {
+  return guard([&] {
+    hamd::Qme& q = Qcfg(h);
+    q.hrchy_filter = filter;
+    q.hrchy_filter_user = filter ? user : nullptr;
+    q.ready = false;
+    return HEOM_OK;
+  });
+}
+
 int heom_set_param(heom_handle h) {
   return guard([&] {
     Qcfg(h).set_param();

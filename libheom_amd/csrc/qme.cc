@@ -54,7 +54,7 @@ inline std::string key_of(const int* v, int K) {
 }
 }  // namespace
 
-Hierarchy build_hierarchy(int K, int max_depth) {
+Hierarchy build_hierarchy(int K, int max_depth, heom_hrchy_filter filter, void* user) {
   Hierarchy hs;
   hs.n_dim = K;
   struct Item {
@@ -75,6 +75,7 @@ Hierarchy build_hierarchy(int K, int max_depth) {
       for (int k = it.last; k < K; ++k) {
         Item nx{it.idx, k, it.depth + 1};
         ++nx.idx[k];
+        if (filter && !filter(nx.idx.data(), K, nx.depth, user)) continue;
         queue.push_back(std::move(nx));
       }
     }
@@ -262,7 +263,7 @@ void Qme::set_param() {
   if (K == 0) throw HeomError{HEOM_ERR_ARG, "set_param: no bath modes"};
   if (max_depth < 0) throw HeomError{HEOM_ERR_ARG, "set_param: max_depth < 0"};
 
-  hs = build_hierarchy(K, max_depth);
+  hs = build_hierarchy(K, max_depth, hrchy_filter, hrchy_filter_user);
   const int nh = hs.n_hrchy;
   if (static_cast<long long>(nh) * N2 > (1LL << 31) - 1)
     throw HeomError{HEOM_ERR_UNSUPPORTED, "set_param: main_size exceeds 2^31"};

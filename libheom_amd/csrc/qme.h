@@ -49,7 +49,10 @@ struct Hierarchy {
 // Breadth-first enumeration with the reference's visiting rule
 // (hrchy_space.cc:137-231): pop an index, then push index + e_k for every
 // k >= the mode last incremented, while the total depth stays <= max_depth.
-Hierarchy build_hierarchy(int n_dim, int max_depth);
+// filter (hrchy_space.h:29-36): an index is enumerated only if it returns
+// non-zero (hrchy_space.cc:183-187); null: every index up to max_depth.
+Hierarchy build_hierarchy(int n_dim, int max_depth, heom_hrchy_filter filter = nullptr,
+                          void* user = nullptr);
 
 struct Closure;
 
@@ -66,6 +69,8 @@ class Qme {
   bool H_set = false;
   std::vector<NoiseParams> noise;
   bool ready = false;
+  heom_hrchy_filter hrchy_filter = nullptr;
+  void* hrchy_filter_user = nullptr;
 
   // heom::set_param products
   Hierarchy hs;

@@ -229,6 +229,9 @@ struct Model {
 
   vector<cplx> temp;  // 3*N2 per thread
   bool ready = false;
+  // hrchy_filter of alloc_hrchy_space (hrchy_space.h:29-36); null: none
+  int (*filter)(const int*, int, int, void*) = nullptr;
+  void* filter_user = nullptr;
 };
 
 // calc_multicombination / calc_hrchy_element_count (hrchy_space.cc:36-71).
@@ -266,7 +269,10 @@ void alloc_hrchy(Model& m) {
     for (int k = last; k < K; ++k) {
       ++index[k];
       int depth = std::accumulate(index.begin(), index.end(), 0);
-      if (depth <= m.max_depth) {
+      // hrchy_space.cc:183-184: depth <= max_depth && (!filter_flag || hrchy_filter(...))
+      const bool pass =
+          depth <= m.max_depth && (!m.filter || m.filter(index.data(), K, depth, m.filter_user));
+      if (pass) {
         next.push(index);
         k_last.push(k);
       }
@@ -751,6 +757,13 @@ void* orc_create(int kind, int n_level, int max_depth, int n_threads) {
 }
 
 void orc_destroy(void* p) { delete static_cast<Model*>(p); }
+
+void orc_set_filter(void* p, int (*filter)(const int*, int, int, void*), void* user) {
+  auto* m = static_cast<Model*>(p);
+  m->filter = filter;
+  m->filter_user = user;
+  m->ready = false;
+}
 
 void orc_set_options(void* p, int parity_q1, int sqrt_norm) {
   auto* m = static_cast<Model*>(p);

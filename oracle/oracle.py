@@ -15,6 +15,10 @@ import os
 
 import numpy as np
 
+# int filter(const int* index, int n_dim, int depth, void* user)
+FILTER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                          ctypes.c_void_p)
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libheom_oracle.so")
 _REF_PATH = os.path.join(_HERE, "_ref", "libhrchy_ref.so")
@@ -49,6 +53,7 @@ def lib():
         L.orc_create.argtypes = [ip, ip, ip, ip]
         L.orc_destroy.argtypes = [vp]
         L.orc_set_options.argtypes = [vp, ip, ip]
+        L.orc_set_filter.argtypes = [vp, FILTER, vp]
         L.orc_set_threads.argtypes = [vp, ip]
         L.orc_set_H.argtypes = [vp, vp]
         L.orc_alloc_noises.argtypes = [vp, ip]
@@ -123,6 +128,14 @@ class Oracle:
 
     def set_threads(self, n):
         self._L.orc_set_threads(self.h, n)
+
+    def set_filter(self, fn):
+        """hrchy_filter(index ndarray, depth) -> bool (hrchy_space.h:29-36); before load()."""
+        cb = FILTER(lambda idx, n, depth, user:
+                    1 if fn(np.ctypeslib.as_array(idx, shape=(n,)).copy(), depth) else 0)
+        self._keep.append(cb)
+        self._L.orc_set_filter(self.h, cb, None)
+        return self
 
     def set_H(self, H):
         H = _c128(H, (self.N, self.N))
@@ -209,6 +222,25 @@ class Oracle:
         rc = self._L.orc_solve(self.h, solver, _ptr(rho), _ptr(t_list), n_t, dt, atol, rtol,
                                _ptr(rs), ctypes.byref(dts), ctypes.byref(st))
         return rho, rs, st, dts.value, rc
+
+
+def ref_hrchy_filtered(n_dim, max_depth, fn):
+    """Reference src/hrchy_space.cc enumeration with an hrchy_filter."""
+    if not os.path.exists(_REF_PATH):
+        return None
+    L = ctypes.CDLL(_REF_PATH)
+    L.ref_hrchy_filtered.restype = ctypes.c_int
+    L.ref_hrchy_filtered.argtypes = [ctypes.c_int, ctypes.c_int, FILTER, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_int]
+    cb = FILTER(lambda idx, n, depth, user:
+                1 if fn(np.ctypeslib.as_array(idx, shape=(n,)).copy(), depth) else 0)
+    nh = L.ref_hrchy_filtered(n_dim, max_depth, cb, None, None, None, None, 0)
+    n = np.zeros((nh, n_dim), np.int32)
+    p1 = np.zeros((nh + 1, n_dim), np.int32)
+    m1 = np.zeros((nh + 1, n_dim), np.int32)
+    L.ref_hrchy_filtered(n_dim, max_depth, cb, None, _ptr(n), _ptr(p1), _ptr(m1), nh)
+    return n, p1, m1
 
 
 def ref_hrchy(n_dim, max_depth):

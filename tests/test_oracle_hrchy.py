@@ -76,3 +76,32 @@ def test_every_multi_index_present():
     n, _, _ = oracle_tables(K, L)
     want = {c for c in itertools.product(range(L + 1), repeat=K) if sum(c) <= L}
     assert {tuple(r) for r in n} == want
+
+
+# hrchy_filter (hrchy_space.h:29-36): pinned against the reference's own
+# alloc_hrchy_space compiled from src/hrchy_space.cc
+FILTERS = {
+    "max occupation 2": lambda n, d: n.max() <= 2,
+    "weighted depth": lambda n, d: int((n * (np.arange(n.size) + 1)).sum()) <= 7,
+    # rejects the intermediate index (1, 0, ...): its canonical descendants vanish too
+    "prune through an intermediate": lambda n, d: not (n[0] == 1 and n[1:].sum() == 0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FILTERS))
+@pytest.mark.parametrize("K,L", [(3, 5), (4, 4), (6, 3)])
+def test_filtered_hierarchy_matches_reference(name, K, L):
+    fn = FILTERS[name]
+    ref = O.ref_hrchy_filtered(K, L, fn)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (no /root/reference)")
+    m = configs.random_model(2, 1, K, L, seed=K, dense_v=False)
+    o = O.Oracle(O.HILB, 2, L).set_filter(fn).load(m)
+    n, p1, m1 = o.tables()
+    assert np.array_equal(n, ref[0]) and np.array_equal(p1, ref[1])
+    assert np.array_equal(m1, ref[2])
+    full = O.Oracle(O.HILB, 2, L).load(m).tables()[0]
+    assert len(n) < len(full)
+    keep = {tuple(r) for r in full}
+    assert all(tuple(r) in keep for r in n)                # a subset of the full hierarchy
+    assert all(fn(r, int(r.sum())) for r in n[1:])        # every enumerated index passes

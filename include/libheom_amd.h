@@ -44,9 +44,10 @@ extern "C" {
                              hi + lo bf16 parts, hi*hi + hi*lo + lo*hi
                              accumulated in FP32 (matrix-core path only) */
 
-/* solvers (solver/rk4.h, solver/rkdp.h) */
+/* solvers (solver/rk4.h, solver/rkdp.h, solver/lsrk4.h) */
 #define HEOM_RK4 0
 #define HEOM_RKDP 1
+#define HEOM_LSRK4 2
 
 typedef struct heom_qme* heom_handle;
 typedef struct heom_solver* heom_solver_handle;

@@ -272,6 +272,14 @@ class rkdp : public solver_base<dtype, order, engine> {
   bool adaptive() const override { return true; }
 };
 
+// solver/lsrk4.h
+template <typename dtype, order_t order, typename engine>
+class lsrk4 : public solver_base<dtype, order, engine> {
+ public:
+  int kind() const override { return HEOM_LSRK4; }
+  bool adaptive() const override { return false; }
+};
+
 template <typename dtype, order_t order, typename engine>
 class qme_solver {
  public:

@@ -28,7 +28,7 @@ import numpy as np
 from . import _capi
 from ._capi import HeomError
 
-__all__ = ["hip", "heom_hilb", "heom_liou", "heom_ado", "heom_batch", "rk4", "rkdp",
+__all__ = ["hip", "heom_hilb", "heom_liou", "heom_ado", "heom_batch", "rk4", "rkdp", "lsrk4",
            "qme_solver",
            "HeomError", "row_major", "col_major"]
 
@@ -323,6 +323,11 @@ class rk4:
 class rkdp:
     """Dormand-Prince 5(4) (solver/rkdp.h); kwargs: dt (initial), atol, rtol."""
     _kind = _capi.HEOM_RKDP
+
+
+class lsrk4:
+    """Low-storage four-stage RK (solver/lsrk4.h); kwargs: dt."""
+    _kind = _capi.HEOM_LSRK4
 
 
 class qme_solver:

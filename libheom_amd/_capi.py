@@ -20,7 +20,7 @@ HEOM_ERR_NOMEM = 6
 
 HEOM_HILB, HEOM_LIOU, HEOM_ADO = 0, 1, 2
 HEOM_C128, HEOM_C64, HEOM_C64_BF16X3 = 0, 1, 2
-HEOM_RK4, HEOM_RKDP = 0, 1
+HEOM_RK4, HEOM_RKDP, HEOM_LSRK4 = 0, 1, 2
 HEOM_CLOSURE_PER_ADO, HEOM_CLOSURE_GLOBAL = 0, 1
 
 # every symbol declared in include/libheom_amd.h

@@ -61,7 +61,7 @@ void StageSpec::combo(const std::vector<std::pair<int, double>>& terms, double c
 
 Solver::Solver(Qme* q_, int kind_) : q(q_), kind(kind_) {
   if (!q->ready) throw HeomError{HEOM_ERR_STATE, "solver: set_param has not been called"};
-  if (kind != HEOM_RK4 && kind != HEOM_RKDP)
+  if (kind != HEOM_RK4 && kind != HEOM_RKDP && kind != HEOM_LSRK4)
     throw HeomError{HEOM_ERR_ARG, "solver: unknown solver kind"};
   HAMD_HIP(hipSetDevice(q->opt.device));
   in_bytes = q->lo() ? 4 : 8;
@@ -76,15 +76,18 @@ Solver::Solver(Qme* q_, int kind_) : q(q_), kind(kind_) {
     slo[s] = is_lo;
   };
   mk(SL_Y, z128b, 0);
-  if (kind == HEOM_RKDP) mk(SL_YNEW, z128b, 0);
+  // Dormand-Prince: the y / y_new pair; lsrk4: the state alternates between them
+  if (kind != HEOM_RK4) mk(SL_YNEW, z128b, 0);
   if (lo) {
     mk(SL_Y32, sinb, 1);
-    if (kind == HEOM_RKDP) mk(SL_YNEW32, sinb, 1);
+    if (kind != HEOM_RK4) mk(SL_YNEW32, sinb, 1);
   }
-  mk(SL_SA, sinb, lo);
-  mk(SL_SB, sinb, lo);
-  if (kind == HEOM_RK4) {
-    mk(SL_ACC, z128b, 0);
+  if (kind != HEOM_LSRK4) {
+    mk(SL_SA, sinb, lo);
+    mk(SL_SB, sinb, lo);
+  }
+  if (kind == HEOM_RK4 || kind == HEOM_LSRK4) {
+    mk(SL_ACC, z128b, 0);  // rk4: the accumulator; lsrk4: buff
   } else {
     for (int k = 0; k < 6; ++k) mk(SL_K1 + k, sinb, lo);
   }
@@ -177,6 +180,29 @@ void Solver::build_stages(double atol, double rtol) {
     a.in_slot = SL_SA;
     a.combo({{SL_ACC, 1.0}}, 1.0 / 6.0, SL_Y, lo ? SL_Y32 : -1);
     specs.push_back(a);
+  } else if (kind == HEOM_LSRK4) {
+    // lsrk4.h:33-48: buff = beta*buff + k (k = -dt*D) with beta = 0, -1, -1, -1,
+    // then rho += c*buff, c = 1/3, 3/4, 2/3, 1/4.  The state alternates between
+    // y and y_new (neighbour reads never see a half-written state); rho + c*buff
+    // is formed as rho + c*beta*buff + c*k from the operands before the update.
+    const double beta[4] = {0.0, -1.0, -1.0, -1.0};
+    const double c[4] = {1.0 / 3.0, 3.0 / 4.0, 2.0 / 3.0, 1.0 / 4.0};
+    const int src[4] = {SL_Y, SL_YNEW, SL_Y, SL_YNEW};
+    const int dst[4] = {SL_YNEW, SL_Y, SL_YNEW, SL_Y};
+    const int src32[4] = {SL_Y32, SL_YNEW32, SL_Y32, SL_YNEW32};
+    for (int s = 0; s < 4; ++s) {
+      StageSpec a;
+      a.in_slot = lo ? src32[s] : src[s];
+      if (beta[s] == 0.0) {
+        a.combo({{src[s], 1.0}}, c[s], dst[s], lo ? src32[(s + 1) % 4] : -1);
+        a.combo({}, 1.0, SL_ACC);
+      } else {
+        a.combo({{src[s], 1.0}, {SL_ACC, c[s] * beta[s]}}, c[s], dst[s],
+                lo ? src32[(s + 1) % 4] : -1);
+        a.combo({{SL_ACC, beta[s]}}, 1.0, SL_ACC);
+      }
+      specs.push_back(a);
+    }
   } else {
     // Dormand-Prince: stage i (0-based) evaluates k_i at its input and writes
     // the input of stage i+1 (rkdp.h:111-127); stage 5 also forms the new
@@ -344,7 +370,8 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
   for (int i = 0; i + 1 < n_t; ++i)
     if (!(t_list[i + 1] >= t_list[i]))
       throw HeomError{HEOM_ERR_ARG, "solve: t_list must be non-decreasing"};
-  if (kind == HEOM_RK4 && !(dt > 0.0)) throw HeomError{HEOM_ERR_ARG, "rk4: dt must be > 0"};
+  if (kind != HEOM_RKDP && !(dt > 0.0))
+    throw HeomError{HEOM_ERR_ARG, "rk4/lsrk4: dt must be > 0"};
   if (kind == HEOM_RKDP) {
     if (!(dt > 0.0) && dt_save <= 0.0) throw HeomError{HEOM_ERR_ARG, "rkdp: dt must be > 0"};
     if (!(atol >= 0.0) || !(rtol >= 0.0) || (atol == 0.0 && rtol == 0.0))
@@ -359,7 +386,7 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
   HAMD_HIP(hipMemcpyAsync(P[SL_Y].p, rho_dev, bytes, hipMemcpyDeviceToDevice, st));
   if (in_bytes == 4) launch_narrow(P[SL_Y32].p, P[SL_Y].p, ms, st);
   int gave_up = 0;
-  if (kind == HEOM_RK4) {
+  if (kind != HEOM_RKDP) {  // fixed step: rk4, lsrk4
     for (int i = 0; i < n_t - 1; ++i) {
       on_output(i, P[SL_Y].p);
       rk4_interval(t_list[i], t_list[i + 1], dt);

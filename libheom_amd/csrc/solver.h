@@ -49,7 +49,7 @@ class Solver {
   ~Solver();
 
   Qme* q;
-  int kind;        // HEOM_RK4 / HEOM_RKDP
+  int kind;        // HEOM_RK4 / HEOM_RKDP / HEOM_LSRK4
   int in_bytes;    // real size of the operator arithmetic and stage inputs (8 or 4)
   long long ms;    // main_size
   double dt_save = -1.0;
@@ -66,7 +66,7 @@ class Solver {
   // Average duration (ms) of each stage kernel over `reps` replays.
   void time_stages(int reps, std::vector<double>& ms_out);
   void stage_bytes(std::vector<double>& bytes_out);
-  int n_stages() const { return kind == HEOM_RK4 ? 4 : 7; }
+  int n_stages() const { return kind == HEOM_RKDP ? 7 : 4; }
 
  private:
   static constexpr int kSlots = 16;

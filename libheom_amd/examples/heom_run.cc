@@ -9,7 +9,7 @@
 //   form(0 hilb|1 liou|2 ado) N max_depth n_noise
 //   H[N*N]
 //   per noise: len V[N*N] gamma[len*len] phi_0[len] sigma[len] S[len*len] A[len*len] s_delta
-//   solver(0 rk4|1 rkdp) dt atol rtol n_t t_list[n_t]
+//   solver(0 rk4|1 rkdp|2 lsrk4) dt atol rtol n_t t_list[n_t]
 //   rho0[main_size]
 // Output: one line per output time: t followed by rho_S (N*N complex, re im).
 #include <cstdio>
@@ -86,6 +86,8 @@ int main(int argc, char** argv) {
     std::unique_ptr<solver_base<dtype, row_major, engine>> solver;
     if (solver_kind == 0)
       solver.reset(new rk4<dtype, row_major, engine>());
+    else if (solver_kind == 2)
+      solver.reset(new lsrk4<dtype, row_major, engine>());
     else
       solver.reset(new rkdp<dtype, row_major, engine>());
     qme_solver<dtype, row_major, engine> qs(&eng, qme.get(), solver.get());

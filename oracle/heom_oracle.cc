@@ -658,9 +658,24 @@ void rk4_step(Model& m, cplx* rho, double dt, vector<cplx>& buf, Stats& st) {
   st.n_steps += 1;
 }
 
+// lsrk4::solve_fixed_step (lsrk4.h:33-48): buff = beta*buff - dt*D(rho) with
+// beta = 0, -1, -1, -1, each followed by rho += c*buff, c = 1/3, 3/4, 2/3, 1/4.
+void lsrk4_step(Model& m, cplx* rho, double dt, vector<cplx>& buf, Stats& st) {
+  const long ms = static_cast<long>(m.n_hrchy) * m.N2;
+  cplx* b = &buf[0];
+  const double beta[4] = {0.0, -1.0, -1.0, -1.0};
+  const double c[4] = {1.0 / 3.0, 3.0 / 4.0, 2.0 / 3.0, 1.0 / 4.0};
+  for (int s = 0; s < 4; ++s) {
+    calc_diff(m, b, rho, dt, beta[s]);
+    vaxpy(ms, c[s], b, rho);
+  }
+  st.n_rhs += 4;
+  st.n_steps += 1;
+}
+
 // fixed_step_size_solver::solve_1 (fixed_step_size_solver.h:29-52).
 void fixed_solve_1(Model& m, cplx* rho, double t0, double t1, double dt, vector<cplx>& buf,
-                   Stats& st) {
+                   Stats& st, bool low_storage = false) {
   double t = t0;
   double dt_1 = dt;
   bool brk = false;
@@ -669,7 +684,10 @@ void fixed_solve_1(Model& m, cplx* rho, double t0, double t1, double dt, vector<
       dt_1 = t1 - t;
       brk = true;
     }
-    rk4_step(m, rho, dt_1, buf, st);
+    if (low_storage)
+      lsrk4_step(m, rho, dt_1, buf, st);
+    else
+      rk4_step(m, rho, dt_1, buf, st);
     t += dt_1;
     if (brk) break;
   }
@@ -881,6 +899,15 @@ void orc_rk4_step(void* p, double* rho, double dt) {
   rk4_step(*m, reinterpret_cast<cplx*>(rho), dt, buf, st);
 }
 
+// One lsrk4 step (lsrk4.h:33-48) in place.
+void orc_lsrk4_step(void* p, double* rho, double dt) {
+  auto* m = static_cast<Model*>(p);
+  const long ms = static_cast<long>(m->n_hrchy) * m->N2;
+  vector<cplx> buf(ms);
+  Stats st;
+  lsrk4_step(*m, reinterpret_cast<cplx*>(rho), dt, buf, st);
+}
+
 // One rkdp adaptive step (rkdp.h:81-179) in place; t and dt are in/out.
 int orc_rkdp_step(void* p, double* rho, double* t, double t_bound, double* dt, double atol,
                   double rtol, orc_stats* out) {
@@ -902,13 +929,14 @@ int orc_rkdp_step(void* p, double* rho, double* t, double t_bound, double* dt, d
 // solver_base::solve (solver_base.h:64-79) with the callback replaced by a
 // record of rho_S = rho[0:N2] at every t_list point (what the callback sees).
 // solver: 0 = rk4 (fixed_step_size_solver.h:29-52), 1 = rkdp
-// (adaptive_step_size_solver.h:32-56).  dt_inout returns dt_save for rkdp.
+// (adaptive_step_size_solver.h:32-56), 2 = lsrk4.  dt_inout returns dt_save
+// for rkdp.
 int orc_solve(void* p, int solver, double* rho, const double* t_list, int n_t, double dt,
               double atol, double rtol, double* rho_s_out, double* dt_inout, orc_stats* out) {
   auto* m = static_cast<Model*>(p);
   const long ms = static_cast<long>(m->n_hrchy) * m->N2;
   cplx* r = reinterpret_cast<cplx*>(rho);
-  vector<cplx> buf((solver == 0 ? 4 : 9) * ms);
+  vector<cplx> buf((solver == 1 ? 9 : 4) * ms);
   Stats st;
   double dt_save = (dt_inout && *dt_inout > 0) ? *dt_inout : -1.0;
   auto record = [&](int i) {
@@ -917,8 +945,8 @@ int orc_solve(void* p, int solver, double* rho, const double* t_list, int n_t, d
   };
   for (int i = 0; i < n_t - 1; ++i) {
     record(i);
-    if (solver == 0) {
-      fixed_solve_1(*m, r, t_list[i], t_list[i + 1], dt, buf, st);
+    if (solver != 1) {
+      fixed_solve_1(*m, r, t_list[i], t_list[i + 1], dt, buf, st, solver == 2);
     } else {
       if (dt_save == -1.0) dt_save = dt;
       double t = t_list[i];

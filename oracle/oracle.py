@@ -24,7 +24,7 @@ _LIB_PATH = os.path.join(_HERE, "_build", "libheom_oracle.so")
 _REF_PATH = os.path.join(_HERE, "_ref", "libhrchy_ref.so")
 
 HILB, LIOU, ADO = 0, 1, 2
-RK4, RKDP = 0, 1
+RK4, RKDP, LSRK4 = 0, 1, 2
 
 _lib = None
 
@@ -76,6 +76,7 @@ def lib():
         L.orc_errnrm1.restype = dp
         L.orc_errnrm1.argtypes = [vp, vp, vp, dp, dp, ctypes.c_long]
         L.orc_rk4_step.argtypes = [vp, vp, dp]
+        L.orc_lsrk4_step.argtypes = [vp, vp, dp]
         L.orc_rkdp_step.restype = ip
         L.orc_rkdp_step.argtypes = [vp, vp, vp, dp, vp, dp, dp, ctypes.POINTER(OrcStats)]
         L.orc_solve.restype = ip
@@ -200,6 +201,11 @@ class Oracle:
     def rk4_step(self, rho, dt):
         rho = _c128(rho).reshape(-1).copy()
         self._L.orc_rk4_step(self.h, _ptr(rho), dt)
+        return rho
+
+    def lsrk4_step(self, rho, dt):
+        rho = _c128(rho).reshape(-1).copy()
+        self._L.orc_lsrk4_step(self.h, _ptr(rho), dt)
         return rho
 
     def rkdp_step(self, rho, t, t_bound, dt, atol, rtol):

@@ -37,11 +37,11 @@ def test_header_compiles_with_plain_gxx():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form,solver", [(0, 0), (0, 1), (1, 0), (2, 1)])
+@pytest.mark.parametrize("form,solver", [(0, 0), (0, 1), (1, 0), (2, 1), (0, 2)])
 def test_heom_run_matches_oracle(tmp_path, form, solver):
     m = configs.spin_boson(max_depth=6)
     t_list = np.linspace(0.0, 2.0, 11)
-    dt, atol, rtol = (0.01, 0.0, 0.0) if solver == 0 else (0.05, 1e-10, 1e-8)
+    dt, atol, rtol = (0.05, 1e-10, 1e-8) if solver == 1 else (0.01, 0.0, 0.0)
     f = tmp_path / "model.txt"
     write_model(str(f), form, m, solver, dt, atol, rtol, t_list, m["rho0"])
     print("heom_run", form, solver, flush=True)
@@ -51,6 +51,6 @@ def test_heom_run_matches_oracle(tmp_path, form, solver):
     assert np.allclose(rows[:, 0], t_list)
     rs = (rows[:, 1::2] + 1j * rows[:, 2::2]).reshape(len(t_list), 2, 2)
     o = O.Oracle([O.HILB, O.LIOU, O.ADO][form], 2, m["max_depth"]).load(m)
-    _, rs_o, _, _, _ = o.solve(m["rho0"], t_list, O.RK4 if solver == 0 else O.RKDP, dt, atol,
+    _, rs_o, _, _, _ = o.solve(m["rho0"], t_list, [O.RK4, O.RKDP, O.LSRK4][solver], dt, atol,
                                rtol)
     assert np.abs(rs - rs_o).max() <= 1e-10

@@ -56,6 +56,23 @@ def test_rk4_step_is_taylor_polynomial():
     assert np.abs(got - want).max() <= 1e-13
 
 
+def test_lsrk4_step_is_taylor_polynomial():
+    """lsrk4.h:33-48 (four stages, buff = beta*buff - dt*D, rho += c*buff) is
+    fourth order: on a linear generator one step is the degree-4 polynomial."""
+    m = small_model()
+    o, A = setup(m)
+    h = 0.05
+    y0 = m["rho0"]
+    hA = h * A
+    want = y0.copy()
+    term = y0.copy()
+    for j in range(1, 5):
+        term = hA @ term / j
+        want = want + term
+    got = o.lsrk4_step(y0, h)
+    assert np.abs(got - want).max() <= 1e-13
+
+
 def dopri_attempt(A, y, h, atol, rtol):
     """One DP5(4) attempt with scipy's RK45 coefficients (k = h f)."""
     a, b, e = RK45.A, RK45.B, RK45.E

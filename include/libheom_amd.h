@@ -34,6 +34,11 @@ extern "C" {
 #define HEOM_HILB 0
 #define HEOM_LIOU 1
 #define HEOM_ADO 2
+/* Redfield QMEs (redfield_hilb.h:24, redfield_liou.h:25): the same parameter
+ * members; max_depth is ignored and main_size = n_level^2 (redfield_hilb.h:39-49,
+ * redfield_liou.h:44-54) */
+#define HEOM_REDFIELD_HILB 3
+#define HEOM_REDFIELD_LIOU 4
 
 /* arithmetic (dtype template parameter: complex128 / complex64) */
 #define HEOM_C128 0  /* FP64 complex everywhere */
@@ -85,7 +90,9 @@ int heom_default_options(heom_options* opt);
 
 /* heom_hilb / heom_liou / heom_ado constructor (heom.h:44-50:
  * heom(int max_depth, int n_inner_threads, int n_outer_threads)); threads are
- * replaced by the device.  kind: HEOM_HILB|LIOU|ADO, dtype: HEOM_C128|C64. */
+ * replaced by the device.  kind: HEOM_HILB|LIOU|ADO, dtype: HEOM_C128|C64.
+ * kind HEOM_REDFIELD_HILB|LIOU: the redfield_hilb / redfield_liou default
+ * constructors (redfield_hilb.h:36-37, redfield_liou.h:41-42); max_depth ignored. */
 int heom_create(int kind, int dtype, int n_level, int max_depth, const heom_options* opt,
                 heom_handle* out);
 int heom_destroy(heom_handle h);
@@ -118,6 +125,14 @@ int heom_main_size(heom_handle h, long long* main_size);     /* heom_hilb.h:45-4
 /* hierarchy tables as built (hrchy_space.h:18-27): n (n_hrchy x n_dim),
  * ptr_p1 / ptr_m1 ((n_hrchy+1) x n_dim, sentinel = n_hrchy).  Any may be NULL. */
 int heom_get_hrchy(heom_handle h, int* n, int* ptr_p1, int* ptr_m1);
+
+/* Redfield set-up products after heom_set_param (redfield.h:43-174): the
+ * relaxation operator Lambda_u = U (U^+ V_u U o Ct_u) U^+ of noise u as an
+ * N x N row-major complex matrix (logical, whatever the storage order), and
+ * the eigenvalues of H (N doubles, in the eigensolver's order).  Either output
+ * may be NULL.  The reference keeps them as the public members Lambda and the
+ * local w of redfield::set_param. */
+int heom_redfield_lambda(heom_handle h, int u, double* lambda, double* eig);
 
 /* calc_diff_impl (qme_base.h:74-79): drho_dev = beta*drho_dev - alpha*D(rho_dev)
  * on device buffers of main_size complex elements of the handle's dtype.

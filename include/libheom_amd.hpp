@@ -250,6 +250,37 @@ class heom_ado : public heom<dtype, order, engine> {
   }
 };
 
+// Redfield QMEs (redfield.h:19-174): the same parameter members; set_param
+// computes Lambda_u from the eigendecomposition of H and the bath correlations
+template <typename dtype, order_t order, typename engine>
+class redfield : public qme_base<dtype, order, engine> {
+ public:
+  // Lambda_u (row-major N x N, logical) after set_param (redfield.h:30, 166)
+  std::vector<dtype> Lambda(int u) const {
+    std::vector<double> buf(2 * static_cast<size_t>(this->n_level) * this->n_level);
+    check(heom_redfield_lambda(this->h_, u, buf.data(), nullptr));
+    std::vector<dtype> out(buf.size() / 2);
+    for (size_t i = 0; i < out.size(); ++i) out[i] = dtype(buf[2 * i], buf[2 * i + 1]);
+    return out;
+  }
+};
+
+template <int n_level_c, typename dtype,
+          template <int, typename, order_t, typename> class matrix_base, order_t order,
+          typename engine>
+class redfield_hilb : public redfield<dtype, order, engine> {
+ public:
+  redfield_hilb() { this->kind_ = HEOM_REDFIELD_HILB; }  // redfield_hilb.h:36-37
+};
+
+template <int n_level_c, typename dtype,
+          template <int, typename, order_t, typename> class matrix_base, order_t order,
+          order_t order_liou, typename engine>
+class redfield_liou : public redfield<dtype, order, engine> {
+ public:
+  redfield_liou() { this->kind_ = HEOM_REDFIELD_LIOU; }  // redfield_liou.h:41-42
+};
+
 template <typename dtype, order_t order, typename engine>
 class solver_base {
  public:

@@ -28,7 +28,8 @@ import numpy as np
 from . import _capi
 from ._capi import HeomError
 
-__all__ = ["hip", "heom_hilb", "heom_liou", "heom_ado", "heom_batch", "rk4", "rkdp", "lsrk4",
+__all__ = ["hip", "heom_hilb", "heom_liou", "heom_ado", "heom_batch", "redfield_hilb",
+           "redfield_liou", "rk4", "rkdp", "lsrk4",
            "qme_solver",
            "HeomError", "row_major", "col_major"]
 
@@ -240,6 +241,41 @@ class heom_liou(_heom):
 class heom_ado(_heom):
     """Global sparse form, one matrix-vector product (heom_ado.h:27)."""
     _kind = _capi.HEOM_ADO
+
+
+class _redfield(_heom):
+    """Redfield QME (redfield.h:19-174): the qme_base parameter members; set_param
+    diagonalises H and builds Lambda_u from the bath correlations.  The state is
+    one N x N density matrix (main_size = N^2)."""
+
+    def __init__(self, dtype="complex128", order=row_major):
+        super().__init__(0, dtype=dtype, order=order)
+
+    def Lambda(self, u):
+        """Lambda_u after set_param (redfield.h:30, 166), N x N."""
+        return self._lambda_eig(u)[0]
+
+    def eigenvalues(self):
+        """Eigenvalues of H from set_param (in the eigensolver's order)."""
+        return self._lambda_eig(0)[1]
+
+    def _lambda_eig(self, u):
+        if not self._ready:
+            raise HeomError(_capi.HEOM_ERR_STATE, "set_param has not been called")
+        lam = np.zeros((self.n_level, self.n_level), np.complex128)
+        w = np.zeros(self.n_level)
+        _capi.check(_capi.load().heom_redfield_lambda(self._h, u, _ptr(lam), _ptr(w)))
+        return lam, w
+
+
+class redfield_hilb(_redfield):
+    """Hilbert-space Redfield form (redfield_hilb.h:24)."""
+    _kind = _capi.HEOM_REDFIELD_HILB
+
+
+class redfield_liou(_redfield):
+    """Liouville-space Redfield form, one sparse superoperator (redfield_liou.h:25)."""
+    _kind = _capi.HEOM_REDFIELD_LIOU
 
 
 class heom_batch:

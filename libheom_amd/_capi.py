@@ -19,6 +19,7 @@ HEOM_ERR_UNSUPPORTED = 5
 HEOM_ERR_NOMEM = 6
 
 HEOM_HILB, HEOM_LIOU, HEOM_ADO = 0, 1, 2
+HEOM_REDFIELD_HILB, HEOM_REDFIELD_LIOU = 3, 4
 HEOM_C128, HEOM_C64, HEOM_C64_BF16X3 = 0, 1, 2
 HEOM_RK4, HEOM_RKDP, HEOM_LSRK4 = 0, 1, 2
 HEOM_CLOSURE_PER_ADO, HEOM_CLOSURE_GLOBAL = 0, 1
@@ -27,7 +28,7 @@ HEOM_CLOSURE_PER_ADO, HEOM_CLOSURE_GLOBAL = 0, 1
 EXPORTS = [
     "heom_last_error", "heom_version", "heom_default_options", "heom_create", "heom_destroy",
     "heom_set_H", "heom_alloc_noises", "heom_set_noise", "heom_set_hrchy_filter", "heom_set_param", "heom_get_n_hrchy",
-    "heom_get_n_dim", "heom_main_size", "heom_get_hrchy", "heom_calc_diff",
+    "heom_get_n_dim", "heom_main_size", "heom_get_hrchy", "heom_redfield_lambda", "heom_calc_diff",
     "heom_calc_diff_host", "heom_batch_create", "heom_batch_layout", "heom_solver_create", "heom_solver_destroy", "heom_solve",
     "heom_solve_dev", "heom_solver_stats", "heom_solver_reset", "heom_solver_set_graphs", "heom_solver_time_stages",
     "heom_solver_stage_bytes", "heom_dev_alloc", "heom_dev_free", "heom_memcpy_h2d",
@@ -96,6 +97,7 @@ def load():
         "heom_get_n_dim": (ip, [vp, P(ip)]),
         "heom_main_size": (ip, [vp, P(llp)]),
         "heom_get_hrchy": (ip, [vp, vp, vp, vp]),
+        "heom_redfield_lambda": (ip, [vp, ip, vp, vp]),
         "heom_calc_diff": (ip, [vp, vp, vp, dp, dp, dp, dp, vp]),
         "heom_calc_diff_host": (ip, [vp, vp, vp, dp, dp, dp, dp]),
         "heom_batch_create": (ip, [vp, ip, P(vp)]),

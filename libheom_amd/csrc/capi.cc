@@ -107,8 +107,10 @@ int heom_create(int kind, int dtype, int n_level, int max_depth, const heom_opti
   return guard([&] {
     if (!out) throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: out is NULL"};
     *out = nullptr;
-    if (kind < HEOM_HILB || kind > HEOM_ADO)
+    if (kind < HEOM_HILB || kind > HEOM_REDFIELD_LIOU)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: unknown kind"};
+    const bool redfield = kind == HEOM_REDFIELD_HILB || kind == HEOM_REDFIELD_LIOU;
+    if (redfield) max_depth = 0;
     if (dtype != HEOM_C128 && dtype != HEOM_C64 && dtype != HEOM_C64_BF16X3)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: unknown dtype"};
     if (n_level < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: n_level < 1"};
@@ -118,8 +120,9 @@ int heom_create(int kind, int dtype, int n_level, int max_depth, const heom_opti
     if (opt) o = *opt;
     if (o.order != 0 && o.order != 1)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: order must be 0 or 1"};
-    if (o.order == 1 && kind != HEOM_HILB)
-      throw hamd::HeomError{HEOM_ERR_UNSUPPORTED, "col_major is provided for heom_hilb only"};
+    if (o.order == 1 && kind != HEOM_HILB && !redfield)
+      throw hamd::HeomError{HEOM_ERR_UNSUPPORTED,
+                            "col_major is provided for heom_hilb and the Redfield forms"};
     auto h = std::make_unique<heom_qme>();
     h->q = std::make_unique<hamd::Qme>(kind, dtype, n_level, max_depth, o);
     *out = h.release();
@@ -228,6 +231,18 @@ int heom_get_hrchy(heom_handle h, int* n, int* p1, int* m1) {
     if (n) std::memcpy(n, q.hs.n.data(), sizeof(int) * q.hs.n.size());
     if (p1) std::memcpy(p1, q.hs.ptr_p1.data(), sizeof(int) * q.hs.ptr_p1.size());
     if (m1) std::memcpy(m1, q.hs.ptr_m1.data(), sizeof(int) * q.hs.ptr_m1.size());
+    return HEOM_OK;
+  });
+}
+
+int heom_redfield_lambda(heom_handle h, int u, double* lambda, double* eig) {
+  return guard([&] {
+    hamd::Qme& q = Qready(h);
+    if (!q.redfield()) throw hamd::HeomError{HEOM_ERR_ARG, "not a Redfield QME"};
+    if (u < 0 || u >= static_cast<int>(q.rf_lambda.size()))
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_redfield_lambda: noise index out of range"};
+    if (lambda) std::memcpy(lambda, q.rf_lambda[u].data(), sizeof(hamd::z128) * q.N2);
+    if (eig) std::memcpy(eig, q.rf_eig.data(), sizeof(double) * q.N);
     return HEOM_OK;
   });
 }

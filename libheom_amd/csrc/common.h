@@ -160,11 +160,11 @@ struct BatchItem {
   int inst, ado;
 };
 
-constexpr int kBatch = 3;  // FormDev::kind of a packed batch
+constexpr int kBatch = 16;  // Qme / FormDev kind of a packed batch (not a public kind)
 
 // Operator tables of one QME in device memory.
 struct FormDev {
-  int kind;  // 0 hilb, 1 liou, 2 ado, 3 batch of hilb
+  int kind;  // public kind (HEOM_HILB .. HEOM_REDFIELD_LIOU) or kBatch
   int N, N2, n_hrchy, n_noise;
   long long main_size;
   const int* link_ptr;   // n_hrchy+1
@@ -206,6 +206,11 @@ struct FormDev {
   // [... + N^2 + e] (imaginary part) added to D; null when no closure is bound
   const float* clos;
   int clos_lo, clos_n;
+  // Redfield: Lambda_u and Lambda_u^+ (n_noise x N x N); rf_T: n_noise x N x N
+  // scratch of the two-pass Hilbert kernel (N > 16), in the operator precision
+  const z128* rf_L;
+  const z128* rf_Ld;
+  void* rf_T;
   // batch
   int n_inst, n_items, batch_grid;
   const BatchInst* inst;

@@ -1132,6 +1132,171 @@ __global__ void __launch_bounds__(256) ado_kernel(const FormDev d, const StageAr
   if (A.do_err) write_partial(A, err_acc);
 }
 
+// ---------------------------------------------------------------------------
+// Redfield QMEs (redfield_hilb.h:66-89, redfield_liou.h:91-102):
+//   D = i[H, rho] + sum_u [V_u, T_u],  T_u = Lambda_u rho - rho Lambda_u^+
+// ---------------------------------------------------------------------------
+// n_level <= 16: the whole state in one workgroup, one lane per element.  rho
+// and H stay in LDS; per noise, Lambda_u, Lambda_u^+ and V_u are staged in
+// LDS, T_u is formed element-wise, and [V_u, T_u] is accumulated after a
+// barrier.
+template <typename R, bool F64, int KP>
+__global__ void __launch_bounds__(256) redfield_small_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  __shared__ C s_rho[256], s_H[256], s_L[256], s_Ld[256], s_V[256], s_T[256];
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const int N = d.N, N2 = d.N2;
+  const int e = threadIdx.x;
+  const bool ok = e < N2;
+  const int i = ok ? e / N : 0, j = ok ? e - (e / N) * N : 0;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  PreT<KP> pv;
+  prefetch<F64, KP>(pv, A, cur, e, ok);
+  if (ok) {
+    s_rho[e] = in[e];
+    s_H[e] = conv<R>(d.H[e]);
+  }
+  __syncthreads();
+  C D{0, 0};
+  if (ok) {
+    C c{0, 0};
+    for (int k = 0; k < N; ++k) {
+      cmac(c, s_H[i * N + k], s_rho[k * N + j]);
+      cmac(c, s_rho[i * N + k], C{-s_H[k * N + j].re, -s_H[k * N + j].im});
+    }
+    D = times_i(c);
+  }
+  for (int u = 0; u < d.n_noise; ++u) {
+    const long long b = static_cast<long long>(u) * N2;
+    if (ok) {
+      s_L[e] = conv<R>(d.rf_L[b + e]);
+      s_Ld[e] = conv<R>(d.rf_Ld[b + e]);
+      s_V[e] = conv<R>(d.V[b + e]);
+    }
+    __syncthreads();
+    if (ok) {
+      C t{0, 0};
+      for (int k = 0; k < N; ++k) {
+        cmac(t, s_L[i * N + k], s_rho[k * N + j]);
+        cmac(t, s_rho[i * N + k], C{-s_Ld[k * N + j].re, -s_Ld[k * N + j].im});
+      }
+      s_T[e] = t;
+    }
+    __syncthreads();
+    if (ok) {
+      for (int k = 0; k < N; ++k) {
+        cmac(D, s_V[i * N + k], s_T[k * N + j]);
+        cmac(D, s_T[i * N + k], C{-s_V[k * N + j].re, -s_V[k * N + j].im});
+      }
+    }
+    __syncthreads();
+  }
+  double err = 0.0;
+  if (ok) err = epilogue<F64, R, KP>(pv, A, cur, e, D, pg.alpha);
+  if (A.do_err) write_partial(A, err);
+}
+
+// n_level > 16, first pass: T_u = Lambda_u rho - rho Lambda_u^+ for every
+// noise, one lane per (u, element); operands come through L1/L2.
+template <typename R>
+__global__ void __launch_bounds__(256) redfield_t_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const C* in = static_cast<const C*>(pick(A.in, pg.cur));
+  C* T = static_cast<C*>(d.rf_T);
+  const int N = d.N, N2 = d.N2;
+  const long long total = static_cast<long long>(d.n_noise) * N2;
+  for (long long x = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; x < total;
+       x += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long u = x / N2;
+    const int e = static_cast<int>(x - u * N2);
+    const int i = e / N, j = e - i * N;
+    const z128* L = d.rf_L + u * N2;
+    const z128* Ld = d.rf_Ld + u * N2;
+    C t{0, 0};
+    for (int k = 0; k < N; ++k) {
+      cmac(t, conv<R>(L[i * N + k]), in[k * N + j]);
+      const C ld = conv<R>(Ld[k * N + j]);
+      cmac(t, in[i * N + k], C{-ld.re, -ld.im});
+    }
+    T[x] = t;
+  }
+}
+
+// n_level > 16, second pass: D = i[H, rho] + sum_u [V_u, T_u] (diagonal V:
+// (v_i - v_j) T_u[i][j]) and the stage epilogue, one lane per element.
+template <typename R, bool F64, int KP>
+__global__ void __launch_bounds__(256) redfield_d_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  const C* T = static_cast<const C*>(d.rf_T);
+  const int N = d.N, N2 = d.N2;
+  double err_acc = 0.0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < N2; e += gridDim.x * blockDim.x) {
+    PreT<KP> pv;
+    prefetch<F64, KP>(pv, A, cur, e, true);
+    const int i = e / N, j = e - i * N;
+    C c{0, 0};
+    for (int k = 0; k < N; ++k) {
+      cmac(c, conv<R>(d.H[i * N + k]), in[k * N + j]);
+      const C h = conv<R>(d.H[k * N + j]);
+      cmac(c, in[i * N + k], C{-h.re, -h.im});
+    }
+    C D = times_i(c);
+    for (int u = 0; u < d.n_noise; ++u) {
+      const long long b = static_cast<long long>(u) * N2;
+      if (d.diag_v) {
+        const C vij = conv<R>(d.V[b + i * N + i]) - conv<R>(d.V[b + j * N + j]);
+        cmac(D, vij, T[b + e]);
+      } else {
+        for (int k = 0; k < N; ++k) {
+          cmac(D, conv<R>(d.V[b + i * N + k]), T[b + k * N + j]);
+          const C v = conv<R>(d.V[b + k * N + j]);
+          cmac(D, T[b + i * N + k], C{-v.re, -v.im});
+        }
+      }
+    }
+    err_acc += epilogue<F64, R, KP>(pv, A, cur, e, D, pg.alpha);
+  }
+  if (A.do_err) write_partial(A, err_acc);
+}
+
+// Sparse superoperator times the state, one 64-lane wavefront per row: the
+// lanes stride over the row's entries (coalesced) and reduce with shuffles;
+// lane 0 runs the epilogue.  Used for redfield_liou, whose rows hold up to
+// N^2 entries (R = L - sum_u Phi_u Theta_u).
+template <typename R, bool F64, int KP>
+__global__ void __launch_bounds__(256) csr_wave_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  const int lane = threadIdx.x & 63;
+  const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const long long n_waves = (static_cast<long long>(gridDim.x) * blockDim.x) >> 6;
+  double err_acc = 0.0;
+  for (long long r = wave; r < d.main_size; r += n_waves) {
+    PreT<KP> pv;
+    prefetch<F64, KP>(pv, A, cur, r, lane == 0);
+    C D{0, 0};
+    for (long long p = d.R_ptr[r] + lane; p < d.R_ptr[r + 1]; p += 64)
+      cmac(D, conv<R>(d.R_val[p]), in[d.R_col[p]]);
+    for (int off = 32; off > 0; off >>= 1) {
+      D.re += __shfl_down(D.re, off, 64);
+      D.im += __shfl_down(D.im, off, 64);
+    }
+    if (lane == 0) err_acc += epilogue<F64, R, KP>(pv, A, cur, r, D, pg.alpha);
+  }
+  if (A.do_err) write_partial(A, err_acc);
+}
+
 __global__ void ctrl_init_kernel(Ctrl* c, double t0, double t_bound, double dt) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (dt < 0.0) dt = c->dt;  // keep dt_save (adaptive_step_size_solver.h:39-41,55)
@@ -1301,8 +1466,41 @@ int launch_hilb(const FormDev& d, const StageArgs& A, hipStream_t st) {
 }
 
 template <typename R, bool F64>
+int launch_redfield(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  const bool small = A.npre <= 4 && !A.do_err;
+  if (d.kind == HEOM_REDFIELD_LIOU) {
+    const int grid = cap_grid((d.main_size + 3) / 4);  // four rows per workgroup
+    if (small)
+      hipLaunchKernelGGL((csr_wave_kernel<R, F64, 4>), dim3(grid), dim3(256), 0, st, d, A);
+    else
+      hipLaunchKernelGGL((csr_wave_kernel<R, F64, kMaxPre>), dim3(grid), dim3(256), 0, st, d, A);
+    return grid;
+  }
+  if (d.N <= 16) {
+    if (small)
+      hipLaunchKernelGGL((redfield_small_kernel<R, F64, 4>), dim3(1), dim3(256), 0, st, d, A);
+    else
+      hipLaunchKernelGGL((redfield_small_kernel<R, F64, kMaxPre>), dim3(1), dim3(256), 0, st, d,
+                         A);
+    return 1;
+  }
+  if (d.rf_T == nullptr) throw HeomError{HEOM_ERR_STATE, "redfield_hilb: scratch missing"};
+  const long long nt = static_cast<long long>(d.n_noise) * d.N2;
+  hipLaunchKernelGGL((redfield_t_kernel<R>), dim3(cap_grid((nt + 255) / 256)), dim3(256), 0, st,
+                     d, A);
+  const int grid = cap_grid((d.N2 + 255) / 256);
+  if (small)
+    hipLaunchKernelGGL((redfield_d_kernel<R, F64, 4>), dim3(grid), dim3(256), 0, st, d, A);
+  else
+    hipLaunchKernelGGL((redfield_d_kernel<R, F64, kMaxPre>), dim3(grid), dim3(256), 0, st, d, A);
+  return grid;
+}
+
+template <typename R, bool F64>
 int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
   if (d.kind == 0) return launch_hilb<R, F64>(d, A, st);
+  if (d.kind == HEOM_REDFIELD_HILB || d.kind == HEOM_REDFIELD_LIOU)
+    return launch_redfield<R, F64>(d, A, st);
   if (d.kind == kBatch) {
     const int grid = d.batch_grid > 0 ? d.batch_grid : 1;
     if (A.npre <= 4 && !A.do_err)

@@ -94,6 +94,12 @@ class Qme {
   long long ado_nnz = 0;
 
   void set_param();
+  bool redfield() const { return kind == HEOM_REDFIELD_HILB || kind == HEOM_REDFIELD_LIOU; }
+  // Redfield set-up products: eigenvalues of H (ascending order not implied)
+  // and Lambda_u (logical N x N row-major, before any col_major transform)
+  std::vector<double> rf_eig;
+  std::vector<std::vector<z128>> rf_lambda;
+  DevBuf d_rf_L, d_rf_Ld, d_rf_T;
   // learned truncation closure (mlp.h), null when none is bound
   std::unique_ptr<Closure> closure;
   void clear_closure();
@@ -133,6 +139,7 @@ class Qme {
   std::vector<z128> R_val_;
 
   void build_links();
+  void set_param_redfield();
   void build_superops(std::vector<int>& ptr, std::vector<int>& col, std::vector<z128>& val,
                       std::vector<std::vector<std::map<int, z128>>>& ops);
   void build_ado(const std::vector<std::vector<std::map<int, z128>>>& ops);

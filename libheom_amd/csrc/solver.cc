@@ -464,8 +464,11 @@ void Solver::stage_bytes(std::vector<double>& out) {
   out.assign(ns, 0.0);
   auto size_of = [&](int slot) { return slo[slot] ? 8.0 : 16.0; };
   double tables = 0.0;
-  if (q->kind == HEOM_ADO) {
+  if (q->kind == HEOM_ADO || q->kind == HEOM_REDFIELD_LIOU) {
     tables = 20.0 * q->ado_nnz + 8.0 * (ms + 1);
+  } else if (q->kind == HEOM_REDFIELD_HILB) {
+    // H, and V_u, Lambda_u, Lambda_u^+ per noise
+    tables = 16.0 * q->N2 * (1.0 + 3.0 * static_cast<double>(q->noise.size()));
   } else {
     const double lrec =
         ((q->kind == HEOM_HILB || q->kind == kBatch) && q->diag_v) ? sizeof(LinkFlat) : sizeof(Link);

@@ -6,7 +6,7 @@
 // and prints rho_S at every output time (what the callback sees).
 //
 // Input (whitespace separated, complex numbers as "re im"):
-//   form(0 hilb|1 liou|2 ado) N max_depth n_noise
+//   form(0 hilb|1 liou|2 ado|3 redfield_hilb|4 redfield_liou) N max_depth n_noise
 //   H[N*N]
 //   per noise: len V[N*N] gamma[len*len] phi_0[len] sigma[len] S[len*len] A[len*len] s_delta
 //   solver(0 rk4|1 rkdp|2 lsrk4) dt atol rtol n_t t_list[n_t]
@@ -52,8 +52,12 @@ int main(int argc, char** argv) {
     qme.reset(new heom_hilb<dynamic, dtype, dense_matrix, row_major, engine>(depth, 1, 1));
   else if (form == 1)
     qme.reset(new heom_liou<dynamic, dtype, sparse_matrix, row_major, row_major, engine>(depth, 1, 1));
-  else
+  else if (form == 2)
     qme.reset(new heom_ado<dynamic, dtype, sparse_matrix, row_major, row_major, engine>(depth, 1, 1));
+  else if (form == 3)
+    qme.reset(new redfield_hilb<dynamic, dtype, dense_matrix, row_major, engine>());
+  else
+    qme.reset(new redfield_liou<dynamic, dtype, sparse_matrix, row_major, row_major, engine>());
   qme->n_level = N;
   qme->n_level_2 = N * N;
   fill(in, qme->H, N, N);

@@ -196,7 +196,7 @@ struct Noise {
   cplx s_delta = 0.0;
 };
 
-enum Kind { HILB = 0, LIOU = 1, ADO = 2 };
+enum Kind { HILB = 0, LIOU = 1, ADO = 2, REDFIELD_HILB = 3, REDFIELD_LIOU = 4 };
 
 struct Model {
   int kind = HILB;
@@ -228,6 +228,10 @@ struct Model {
   vector<cplx> R_val;
 
   vector<cplx> temp;  // 3*N2 per thread
+  // redfield::set_param products (redfield.h:43-174)
+  vector<double> rf_w;                         // eigenvalues of H
+  vector<vector<cplx>> rf_Lambda, rf_Lambda_dgr;  // per noise, N x N
+  SupOp rf_R;                                  // redfield_liou R
   bool ready = false;
   // hrchy_filter of alloc_hrchy_space (hrchy_space.h:29-36); null: none
   int (*filter)(const int*, int, int, void*) = nullptr;
@@ -298,8 +302,216 @@ void alloc_hrchy(Model& m) {
 }
 
 // heom::set_param (heom.h:57-139) plus the form-specific set_param.
+// ---------------------------------------------------------------------------
+// Redfield QMEs (redfield.h, redfield_hilb.h, redfield_liou.h)
+// ---------------------------------------------------------------------------
+
+// Hermitian eigendecomposition standing in for the engine's eig (heev with
+// the upper triangle, linalg_engine_mkl.h.j2:365-387): cyclic Jacobi sweeps on
+// A = U diag(w) U^+, U's columns the eigenvectors.  Each rotation zeroes one
+// off-diagonal pair: with a_pq = |a_pq| e^{i phi}, the unitary
+//   G = diag(1, e^{-i phi}) * [[c, s], [-s, c]]
+// (t = tan(theta) the smaller root of t^2 + 2 tau t - 1 = 0,
+// tau = (a_qq - a_pp) / (2 |a_pq|)) diagonalises the 2 x 2 block.
+void jacobi_eig(int n, vector<cplx> A, vector<double>& w, vector<cplx>& U) {
+  U.assign(static_cast<size_t>(n) * n, 0.0);
+  for (int i = 0; i < n; ++i) U[static_cast<size_t>(i) * n + i] = 1.0;
+  auto a = [&](int i, int j) -> cplx& { return A[static_cast<size_t>(i) * n + j]; };
+  double nrm = 0.0;
+  for (const cplx& x : A) nrm += std::norm(x);
+  nrm = std::sqrt(nrm);
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q)
+        if (p != q) off += std::norm(a(p, q));
+    if (std::sqrt(off) <= 1e-16 * (nrm > 0 ? nrm : 1.0)) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double r = std::abs(a(p, q));
+        if (r == 0.0) continue;
+        const cplx e = a(p, q) / r;  // e^{i phi}
+        const double tau = (a(q, q).real() - a(p, p).real()) / (2.0 * r);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::abs(tau) + std::hypot(1.0, tau));
+        const double c = 1.0 / std::hypot(1.0, t), sn = t * c;
+        const cplx gpp = c, gpq = sn, gqp = -sn * std::conj(e), gqq = c * std::conj(e);
+        for (int k = 0; k < n; ++k) {  // A <- A G
+          const cplx x = a(k, p), y = a(k, q);
+          a(k, p) = x * gpp + y * gqp;
+          a(k, q) = x * gpq + y * gqq;
+        }
+        for (int k = 0; k < n; ++k) {  // A <- G^+ A
+          const cplx x = a(p, k), y = a(q, k);
+          a(p, k) = std::conj(gpp) * x + std::conj(gqp) * y;
+          a(q, k) = std::conj(gpq) * x + std::conj(gqq) * y;
+        }
+        for (int k = 0; k < n; ++k) {  // U <- U G
+          cplx& x = U[static_cast<size_t>(k) * n + p];
+          cplx& y = U[static_cast<size_t>(k) * n + q];
+          const cplx x0 = x, y0 = y;
+          x = x0 * gpp + y0 * gqp;
+          y = x0 * gpq + y0 * gqq;
+        }
+        a(p, q) = a(q, p) = 0.0;
+        a(p, p) = a(p, p).real();
+        a(q, q) = a(q, q).real();
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; ++i) w[i] = a(i, i).real();
+}
+
+// lu / lu_solve of the nil engine for row-major operands
+// (linalg_engine_nil.h.j2:323-389 Doolittle with row pivoting, 483-533
+// substitutions).  As in the reference, the pivot search compares against the
+// original diagonal magnitude (the running maximum it declares inside the
+// loop shadows the outer one), so the last row with a larger entry is taken,
+// and a column whose original diagonal is zero is not eliminated.
+vector<cplx> nil_lu_solve(int n, const vector<cplx>& A, const vector<cplx>& b) {
+  vector<vector<cplx>> U(n, vector<cplx>(n, 0.0));
+  vector<vector<char>> Unz(n, vector<char>(n, 0));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      U[i][j] = A[static_cast<size_t>(i) * n + j];
+      Unz[i][j] = U[i][j] != cplx(0.0);
+    }
+  vector<int> pivot(n);
+  std::iota(pivot.begin(), pivot.end(), 0);
+  for (int i = 0; i < n - 1; ++i) {
+    const double mx = std::abs(U[i][i]);
+    int swap = -1;
+    for (int j = i + 1; j < n; ++j)
+      if (Unz[j][i] && std::abs(U[j][i]) > mx) swap = j;
+    if (swap != -1) {
+      std::swap(pivot[i], pivot[swap]);
+      std::swap(U[i], U[swap]);
+      std::swap(Unz[i], Unz[swap]);
+    }
+    if (mx == 0.0) continue;
+    for (int j = i + 1; j < n; ++j) {
+      if (!Unz[j][i]) continue;
+      U[j][i] /= U[i][i];
+      for (int k = i + 1; k < n; ++k)
+        if (Unz[i][k]) {
+          U[j][k] -= U[j][i] * U[i][k];
+          Unz[j][k] = 1;
+        }
+    }
+  }
+  vector<cplx> y(n), x(n);
+  for (int i = 0; i < n; ++i) {
+    y[i] = b[pivot[i]];
+    for (int j = 0; j < i; ++j)
+      if (Unz[i][j]) y[i] -= U[i][j] * y[j];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    x[i] = y[i];
+    for (int j = i + 1; j < n; ++j)
+      if (Unz[i][j]) x[i] -= U[i][j] * x[j];
+    x[i] /= U[i][i];
+  }
+  return x;
+}
+
+// redfield::correlation (redfield.h:33-41):
+// sigma^T C (gamma - i*omega)^-1 phi_0 + s_delta with C = S + iA.
+cplx rf_correlation(const Noise& z, const vector<cplx>& sTC, double omega) {
+  if (z.len == 0) return z.s_delta;
+  vector<cplx> M = z.gamma;
+  for (int k = 0; k < z.len; ++k) M[static_cast<size_t>(k) * z.len + k] -= I1 * omega;
+  const vector<cplx> x = nil_lu_solve(z.len, M, z.phi0);
+  cplx c = 0.0;
+  for (int k = 0; k < z.len; ++k) c += sTC[k] * x[k];
+  return c + z.s_delta;
+}
+
+// redfield::set_param (redfield.h:43-174) and the form set-ups
+// (redfield_hilb.h:51-64, redfield_liou.h:56-89).
+void set_param_redfield(Model& m) {
+  const int N = m.N, N2 = m.N2, nn = static_cast<int>(m.noise.size());
+  m.n_dim = 0;
+  m.n_hrchy = 1;
+  vector<cplx> Hh(N2);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j)
+      Hh[i * N + j] = i <= j ? m.H[i * N + j] : std::conj(m.H[j * N + i]);
+  vector<cplx> U;
+  jacobi_eig(N, Hh, m.rf_w, U);
+  vector<cplx> Ud(N2);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) Ud[j * N + i] = std::conj(U[i * N + j]);
+  m.rf_Lambda.assign(nn, vector<cplx>(N2));
+  m.rf_Lambda_dgr.assign(nn, vector<cplx>(N2));
+  vector<cplx> tmp(N2), Vv(N2);
+  for (int s = 0; s < nn; ++s) {
+    const Noise& z = m.noise[s];
+    // sigma_T_C = sigma^T C by gevm (redfield.h:68-75, linalg_engine_nil.h.j2:169-192)
+    vector<cplx> sTC(z.len, 0.0);
+    for (int i = 0; i < z.len; ++i)
+      for (int j = 0; j < z.len; ++j)
+        sTC[j] += z.sigma[i] *
+                  (z.S[static_cast<size_t>(i) * z.len + j] + I1 * z.A[static_cast<size_t>(i) * z.len + j]);
+    // utf: V_v = U^+ (V U) (linalg_engine_mkl.h.j2:389-419)
+    gemm(N, 1.0, z.V.data(), U.data(), 0.0, tmp.data());
+    gemm(N, 1.0, Ud.data(), tmp.data(), 0.0, Vv.data());
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j)
+        Vv[i * N + j] *= rf_correlation(z, sTC, m.rf_w[j] - m.rf_w[i]);  // omega_ji
+    // utb: Lambda = U (Lambda_v U^+) (linalg_engine_mkl.h.j2:421-450)
+    gemm(N, 1.0, Vv.data(), Ud.data(), 0.0, tmp.data());
+    gemm(N, 1.0, U.data(), tmp.data(), 0.0, m.rf_Lambda[s].data());
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) m.rf_Lambda_dgr[s][j * N + i] = std::conj(m.rf_Lambda[s][i * N + j]);
+  }
+  if (m.kind == REDFIELD_LIOU) {
+    auto mask = [](const vector<cplx>& x) {
+      vector<char> nz(x.size());
+      for (size_t i = 0; i < x.size(); ++i) nz[i] = x[i] != cplx(0.0);
+      return nz;
+    };
+    SupOp L;
+    L.shape(N2);
+    kron_x_1(I1, N, m.H.data(), m.Hnz.data(), L);
+    kron_1_x_T(-I1, N, m.H.data(), m.Hnz.data(), L);
+    m.rf_R.shape(N2);
+    for (int s = 0; s < nn; ++s) {
+      SupOp Phi, Theta;
+      Phi.shape(N2);
+      Theta.shape(N2);
+      kron_x_1(I1, N, m.noise[s].V.data(), m.noise[s].Vnz.data(), Phi);
+      kron_1_x_T(-I1, N, m.noise[s].V.data(), m.noise[s].Vnz.data(), Phi);
+      const vector<char> lnz = mask(m.rf_Lambda[s]), dnz = mask(m.rf_Lambda_dgr[s]);
+      kron_x_1(I1, N, m.rf_Lambda[s].data(), lnz.data(), Theta);
+      kron_1_x_T(-I1, N, m.rf_Lambda_dgr[s].data(), dnz.data(), Theta);
+      supop_gemm(-1.0, Phi, Theta, m.rf_R);  // R = -Phi Theta + R
+    }
+    supop_axpy(1.0, L, m.rf_R);
+    m.rf_R.optimize();
+  }
+  m.ready = true;
+}
+
+// redfield_hilb::calc_diff_impl (redfield_hilb.h:66-89), same gemm sequence.
+void calc_diff_redfield_hilb(Model& m, cplx* drho, const cplx* rho, cplx alpha, cplx beta) {
+  const int N = m.N;
+  vector<cplx> temp(m.N2);
+  gemm(N, -I1 * alpha, m.H.data(), rho, beta, drho);
+  gemm(N, I1 * alpha, rho, m.H.data(), 1.0, drho);
+  for (size_t s = 0; s < m.noise.size(); ++s) {
+    gemm(N, I1, m.rf_Lambda[s].data(), rho, 0.0, temp.data());
+    gemm(N, -I1, rho, m.rf_Lambda_dgr[s].data(), 1.0, temp.data());
+    gemm(N, I1 * alpha, m.noise[s].V.data(), temp.data(), 1.0, drho);
+    gemm(N, -I1 * alpha, temp.data(), m.noise[s].V.data(), 1.0, drho);
+  }
+}
+
 void set_param(Model& m) {
   const int nn = static_cast<int>(m.noise.size());
+  if (m.kind == REDFIELD_HILB || m.kind == REDFIELD_LIOU) {
+    m.N2 = m.N * m.N;
+    set_param_redfield(m);
+    return;
+  }
   m.N2 = m.N * m.N;
   m.n_dim = 0;
   for (auto& z : m.noise) m.n_dim += z.len;
@@ -608,7 +820,11 @@ void calc_diff_ado(Model& m, cplx* drho, const cplx* rho, cplx alpha, cplx beta)
 }
 
 void calc_diff(Model& m, cplx* drho, const cplx* rho, cplx alpha, cplx beta) {
-  if (m.kind == HILB)
+  if (m.kind == REDFIELD_HILB)
+    calc_diff_redfield_hilb(m, drho, rho, alpha, beta);
+  else if (m.kind == REDFIELD_LIOU)
+    supop_gemv(-alpha, m.rf_R, rho, beta, drho);  // redfield_liou.h:91-102
+  else if (m.kind == HILB)
     calc_diff_hilb(m, drho, rho, alpha, beta);
   else if (m.kind == LIOU)
     calc_diff_liou(m, drho, rho, alpha, beta);
@@ -855,6 +1071,29 @@ long orc_main_size(void* p) {
   return static_cast<long>(m->n_hrchy) * m->N2;
 }
 long orc_ado_nnz(void* p) { return static_cast<long>(static_cast<Model*>(p)->R_col.size()); }
+
+// Redfield products: Lambda_u (N x N interleaved) and the eigenvalues of H.
+int orc_redfield_lambda(void* p, int u, double* lambda, double* eig) {
+  auto* m = static_cast<Model*>(p);
+  if (m->kind != REDFIELD_HILB && m->kind != REDFIELD_LIOU) return -1;
+  if (u < 0 || u >= static_cast<int>(m->rf_Lambda.size())) return -1;
+  if (lambda)
+    for (int i = 0; i < m->N2; ++i) {
+      lambda[2 * i] = m->rf_Lambda[u][i].real();
+      lambda[2 * i + 1] = m->rf_Lambda[u][i].imag();
+    }
+  if (eig)
+    for (int i = 0; i < m->N; ++i) eig[i] = m->rf_w[i];
+  return 0;
+}
+
+// Number of stored entries of redfield_liou's R after optimize.
+long orc_redfield_nnz(void* p) {
+  auto* m = static_cast<Model*>(p);
+  long c = 0;
+  for (char x : m->rf_R.nz) c += x;
+  return c;
+}
 
 // n: n_hrchy*n_dim; p1/m1: (n_hrchy+1)*n_dim
 void orc_hrchy_tables(void* p, int* n_out, int* p1_out, int* m1_out) {

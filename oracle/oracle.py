@@ -8,7 +8,9 @@ The oracle is an FP64 restatement of libheom's propagation path:
   heom::set_param (heom.h:57-139), alloc_hrchy_space (hrchy_space.cc:137-231),
   heom_hilb/heom_liou/heom_ado calc_diff_impl (heom_hilb.h:70-215,
   heom_liou.h:121-258, heom_ado.h:191-202), rk4 (rk4.h:40-65),
-  rkdp (rkdp.h:81-179), solver_base::solve (solver_base.h:64-79).
+  rkdp (rkdp.h:81-179), lsrk4 (lsrk4.h:33-48), solver_base::solve
+  (solver_base.h:64-79), and the Redfield QMEs (redfield.h:33-174,
+  redfield_hilb.h:66-89, redfield_liou.h:56-102).
 """
 import ctypes
 import os
@@ -24,6 +26,7 @@ _LIB_PATH = os.path.join(_HERE, "_build", "libheom_oracle.so")
 _REF_PATH = os.path.join(_HERE, "_ref", "libhrchy_ref.so")
 
 HILB, LIOU, ADO = 0, 1, 2
+REDFIELD_HILB, REDFIELD_LIOU = 3, 4
 RK4, RKDP, LSRK4 = 0, 1, 2
 
 _lib = None
@@ -71,6 +74,10 @@ def lib():
         L.orc_ado_nnz.restype = ctypes.c_long
         L.orc_ado_nnz.argtypes = [vp]
         L.orc_hrchy_tables.argtypes = [vp, vp, vp, vp]
+        L.orc_redfield_lambda.restype = ip
+        L.orc_redfield_lambda.argtypes = [vp, ip, vp, vp]
+        L.orc_redfield_nnz.restype = ctypes.c_long
+        L.orc_redfield_nnz.argtypes = [vp]
         L.orc_ngamma_diag.argtypes = [vp, vp]
         L.orc_calc_diff.argtypes = [vp, vp, vp, dp, dp, dp, dp]
         L.orc_errnrm1.restype = dp
@@ -173,6 +180,17 @@ class Oracle:
                            z["s_delta"])
         self.set_param()
         return self
+
+    def redfield_lambda(self, u):
+        """(Lambda_u, eigenvalues of H) of a Redfield model (redfield.h:144-168)."""
+        lam = np.zeros((self.N, self.N), np.complex128)
+        w = np.zeros(self.N)
+        if self._L.orc_redfield_lambda(self.h, u, _ptr(lam), _ptr(w)) != 0:
+            raise ValueError("not a Redfield model or noise index out of range")
+        return lam, w
+
+    def redfield_nnz(self):
+        return int(self._L.orc_redfield_nnz(self.h))
 
     def tables(self):
         K, nh = self.n_dim, self.n_hrchy

@@ -114,3 +114,42 @@ def dense_generator(model, order=None, offdiag_gamma=False, sqrt_norm=False):
                         f = np.sqrt(nj * (nk + 1)) if sqrt_norm else float(nj)
                         M[blk(pos[q])] += f * g[j, k] * I2
     return M, idx
+
+
+def redfield_lambda(H, z):
+    """Redfield relaxation operator from its definition, without diagonalising H:
+
+        Lambda = int_0^inf C(t) e^{-iHt} V e^{iHt} dt + s_delta V,
+        C(t) = sigma^T (S + iA) e^{-gamma t} phi_0.
+
+    Y(t) = (e^{-gamma t} phi_0) kron vec(e^{-iHt} V e^{iHt}) obeys
+    dY/dt = -(gamma kron 1 + 1 kron i ad_H) Y with ad_H X = HX - XH, so its
+    integral is one linear solve in the (bath mode) x (N x N) space.
+    """
+    H = np.asarray(H, np.complex128)
+    N = H.shape[0]
+    V = np.asarray(z["V"], np.complex128)
+    g = np.atleast_2d(np.asarray(z["gamma"], np.complex128))
+    K = g.shape[0]
+    lam = complex(z["s_delta"]) * V
+    if K == 0:
+        return lam
+    M = np.kron(g, np.eye(N * N)) + np.kron(np.eye(K), 1j * _comm(H))
+    rhs = np.kron(np.asarray(z["phi_0"], np.complex128), V.reshape(-1))
+    Y = np.linalg.solve(M, rhs).reshape(K, N * N)
+    c = np.asarray(z["sigma"], np.complex128) @ (np.atleast_2d(z["S"]) + 1j * np.atleast_2d(z["A"]))
+    return lam + (c @ Y).reshape(N, N)
+
+
+def redfield_generator(model):
+    """Dense M with D(rho) = i[H, rho] + sum_u [V_u, Lambda_u rho - rho Lambda_u^+] = M vec(rho)
+    (row-major vectorisation); calc_diff(rho, alpha=1, beta=0) = -M @ rho."""
+    H = np.asarray(model["H"], np.complex128)
+    N = H.shape[0]
+    I = np.eye(N)
+    M = 1j * _comm(H)
+    for z in model["noises"]:
+        lam = redfield_lambda(H, z)
+        T = np.kron(lam, I) - np.kron(I, lam.conj().T.T)
+        M = M + _comm(np.asarray(z["V"], np.complex128)) @ T
+    return M

@@ -258,6 +258,93 @@ def bench_dense64(H, L, reps):
     }
 
 
+def _stage_stats(L, sv, reps):
+    """Mean per-stage device time (ms, HIP events) and algorithmic bytes of a solver."""
+    from libheom_amd import _capi
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    ns = ctypes.c_int()
+    sms, sby = np.zeros(8), np.zeros(8)
+    _capi.check(L.heom_solver_time_stages(sv._s, reps, p(sms), ctypes.byref(ns)))
+    _capi.check(L.heom_solver_stage_bytes(sv._s, p(sby), ctypes.byref(ns)))
+    n = ns.value
+    return float(sms[:n].mean()), float(sby[:n].mean())
+
+
+def bench_lsrk4(H, L, reps):
+    """lsrk4 against rk4 on the 7-site chain (12650 ADOs): the low-storage scheme
+    keeps two state buffers plus the accumulator, so each stage pass moves four
+    state blocks instead of five."""
+    from libheom_amd import configs
+    m = configs.exciton_chain7(t_final=10.0)
+    eng = H.hip(0)
+    q = H.heom_hilb(m["max_depth"]).load(m, eng)
+    out = {"workload": "7-site chain, 12650 ADOs, heom_hilb complex128: one stage pass of "
+                       "each fixed-step solver (dt=0.01 fs)"}
+    for name, cls in (("rk4", H.rk4), ("lsrk4", H.lsrk4)):
+        sv = mk_solver(H, eng, q, cls())
+        sv.solve_rho_s(m["rho0"], m["t_list"][:2], {"dt": 0.01})  # builds the stages
+        st_ms, st_b = _stage_stats(L, sv, reps)
+        out[name] = {"stage_us": 1e3 * st_ms, "bytes_per_stage": st_b,
+                     "hbm_frac": st_b / (st_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    out["lsrk4_stage_speedup"] = out["rk4"]["stage_us"] / out["lsrk4"]["stage_us"]
+    return out
+
+
+def bench_redfield(H, L, reps, cpu=True):
+    """Redfield QMEs (redfield_hilb / redfield_liou).  The states are one density
+    matrix, so the small cases are launch-latency bound and are reported as
+    time-to-solution next to the CPU oracle on the same job; the N=64 case runs
+    the two-pass Hilbert kernels."""
+    from libheom_amd import configs
+    from oracle import oracle as O
+    eng = H.hip(0)
+    out = {}
+
+    def timed(q, solver, m, t_list, kw, okind, oracle_solver):
+        sv = mk_solver(H, eng, q, solver)
+        sv.solve_rho_s(m["rho0"], t_list[:2], kw)  # warm-up
+        sv.reset()
+        t0 = time.perf_counter()
+        _, rs = sv.solve_rho_s(m["rho0"], t_list, kw)
+        el = time.perf_counter() - t0
+        st = sv.stats()
+        st_ms, st_b = _stage_stats(L, sv, reps)
+        r = {"time_to_rho_s_ms": 1e3 * el, "rhs_evals": st["n_rhs"], "stage_us": 1e3 * st_ms,
+             "bytes_per_stage": st_b}
+        if cpu:
+            o = O.Oracle(okind, m["N"], 0, n_threads=1).load(m)
+            t0 = time.perf_counter()
+            _, rs_o, _, _, _ = o.solve(m["rho0"], t_list, oracle_solver, kw["dt"],
+                                       kw.get("atol", 0.0), kw.get("rtol", 0.0))
+            r["cpu_oracle_ms_1_core"] = 1e3 * (time.perf_counter() - t0)
+            r["rho_max_abs_diff_vs_oracle"] = float(np.abs(rs - rs_o).max())
+        return r
+
+    m = configs.exciton_chain7(max_depth=0)
+    m["rho0"] = m["rho0"][:49].copy()
+    kw = {"dt": m["dt"], "atol": m["atol"], "rtol": m["rtol"]}
+    for form, cls, ok in (("hilb", H.redfield_hilb, O.REDFIELD_HILB),
+                          ("liou", H.redfield_liou, O.REDFIELD_LIOU)):
+        out["chain7_%s" % form] = timed(cls().load(m, eng), H.rkdp(), m,
+                                        np.ascontiguousarray(m["t_list"]), kw, ok, O.RKDP)
+    out["chain7_workload"] = ("7-site chain, 7 Drude baths (1 Drude + 2 Pade), Dormand-Prince "
+                              "rtol=1e-8 atol=1e-11, t=0..1000 fs, rho every 10 fs")
+    m = configs.spin_boson(max_depth=0)
+    m["rho0"] = m["rho0"][:4].copy()
+    out["spin_boson_hilb"] = timed(H.redfield_hilb().load(m, eng), H.rk4(), m,
+                                   np.ascontiguousarray(m["t_list"]), {"dt": m["dt"]},
+                                   O.REDFIELD_HILB, O.RK4)
+    out["spin_boson_workload"] = "spin-boson N=2, Drude bath, RK4 dt=0.01 x 2000, rho every 10 steps"
+    m = configs.dense64(max_depth=0, t_final=20.0)
+    m["rho0"] = m["rho0"][:64 * 64].copy()
+    kw = {"dt": m["dt"], "atol": m["atol"], "rtol": m["rtol"]}
+    out["dense64_hilb"] = timed(H.redfield_hilb().load(m, eng), H.rkdp(), m,
+                                np.ascontiguousarray(m["t_list"]), kw, O.REDFIELD_HILB, O.RKDP)
+    out["dense64_workload"] = ("N=64 GOE H, 4 diagonal-projector baths x (1 Drude + 3 Pade), "
+                               "Dormand-Prince rtol=1e-6 to t=20 (two-pass Hilbert kernels)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,6 +354,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-dense", action="store_true")
     ap.add_argument("--no-spin", action="store_true")
+    ap.add_argument("--no-redfield", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every stage kernel directly (profiler runs)")
     ap.add_argument("--reps", type=int, default=20)
@@ -406,6 +494,9 @@ def main():
         out["variants"]["dense64"] = bench_dense64(H, L, 5)
     if not args.no_sweep:
         out["variants"]["batched_sweep_512"] = bench_sweep(H, L, 1, 1, 5)
+    if not args.no_redfield:
+        out["variants"]["redfield"] = bench_redfield(H, L, 20, cpu=not args.no_cpu)
+        out["variants"]["lsrk4_vs_rk4"] = bench_lsrk4(H, L, args.reps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m)
     print(json.dumps(out))

@@ -1,8 +1,9 @@
 """libheom_amd -- MI355X-native HEOM propagation behind libheom's qme API.
 
 Host-side mirror of the reference's C++ template interface (qme_base.h,
-heom_hilb.h, heom_liou.h, heom_ado.h, solver/rk4.h, solver/rkdp.h,
-qme_solver.h) over the C ABI of ``include/libheom_amd.h``:
+heom_hilb.h, heom_liou.h, heom_ado.h, redfield_hilb.h, redfield_liou.h,
+solver/rk4.h, solver/rkdp.h, solver/lsrk4.h, qme_solver.h) over the C ABI of
+``include/libheom_amd.h``:
 
     qme = heom_hilb(max_depth, n_inner_threads, n_outer_threads)
     qme.n_level = N; qme.n_level_2 = N*N

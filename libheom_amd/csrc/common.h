@@ -211,6 +211,9 @@ struct FormDev {
   const z128* rf_L;
   const z128* rf_Ld;
   void* rf_T;
+  // diagonal V: HL = iH + sum_u V_u Lambda_u, HR = -iH + sum_u Lambda_u^+ V_u
+  const z128* rf_HL;
+  const z128* rf_HR;
   // batch
   int n_inst, n_items, batch_grid;
   const BatchInst* inst;

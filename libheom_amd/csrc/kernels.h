@@ -7,6 +7,18 @@ namespace hamd {
 
 constexpr int kMaxGrid = 1 << 18;  // grid cap = partial-sum slots of the error norm
 
+// Redfield Hilbert form: LDS elements (complex, operator precision) the
+// one-workgroup kernel stages -- rho, H or HL, HR (a block kept for both), per
+// noise Lambda_u and Lambda_u^+ (and V_u for dense V), then the diagonals of V
+// (diagonal V) or one T block (dense V).  Problems with n_level <= 16 whose tables fit in
+// kRedfieldLds bytes run in one workgroup; the others take two passes through
+// a scratch block of n_noise N^2 elements.
+constexpr size_t kRedfieldLds = 60 * 1024;  // leaves room for the static LDS of the error norm
+__host__ __device__ inline size_t redfield_lds_elems(int N2, int N, int n_noise, bool diag) {
+  return diag ? static_cast<size_t>(3 + 2 * n_noise) * N2 + static_cast<size_t>(n_noise) * N
+              : static_cast<size_t>(4 + 3 * n_noise) * N2;
+}
+
 // Launch one fused operator pass (calc_diff + epilogue) for the form in `d`.
 // real_bytes: 8 = FP64 operator arithmetic, 4 = FP32.
 // Returns the grid size used (the number of error-norm partials written).

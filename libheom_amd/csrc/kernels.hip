@@ -1136,69 +1136,110 @@ __global__ void __launch_bounds__(256) ado_kernel(const FormDev d, const StageAr
 // Redfield QMEs (redfield_hilb.h:66-89, redfield_liou.h:91-102):
 //   D = i[H, rho] + sum_u [V_u, T_u],  T_u = Lambda_u rho - rho Lambda_u^+
 // ---------------------------------------------------------------------------
-// n_level <= 16: the whole state in one workgroup, one lane per element.  rho
-// and H stay in LDS; per noise, Lambda_u, Lambda_u^+ and V_u are staged in
-// LDS, T_u is formed element-wise, and [V_u, T_u] is accumulated after a
-// barrier.
-template <typename R, bool F64, int KP>
+// One workgroup holds the whole problem (n_level <= 16 and every table within
+// 60 KB of LDS), one lane per element.  Every table is staged into LDS at the
+// start, so that all of their loads are in flight together; then
+//   diagonal V (DIAG): with v_u the diagonal of V_u,
+//     D = HL rho + rho HR - sum_u ( v_u[i] (rho Lambda_u^+)[i][j] + (Lambda_u rho)[i][j] v_u[j] ),
+//     HL = iH + sum_u V_u Lambda_u, HR = -iH + sum_u Lambda_u^+ V_u (formed at set-up),
+//     read from LDS with no further barrier;
+//   dense V: D = i[H, rho], then per noise T_u = Lambda_u rho - rho Lambda_u^+
+//     into LDS, a barrier, and D += [V_u, T_u].
+// LDS (cx<R> elements): rho, H or HL, HR (DIAG), per noise Lambda_u, Lambda_u^+
+// and V_u (dense), then v_u (DIAG) or T (dense).
+template <typename R, bool F64, int KP, bool DIAG>
 __global__ void __launch_bounds__(256) redfield_small_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
-  __shared__ C s_rho[256], s_H[256], s_L[256], s_Ld[256], s_V[256], s_T[256];
+  extern __shared__ __align__(16) unsigned char rf_lds[];
+  C* sm = reinterpret_cast<C*>(rf_lds);
   const Prologue pg = prologue(A);
   if (pg.skip) return;
   const int cur = pg.cur;
-  const int N = d.N, N2 = d.N2;
+  const int N = d.N, N2 = d.N2, nn = d.n_noise;
+  constexpr int per = DIAG ? 2 : 3;  // tables per noise
+  C* s_rho = sm;
+  C* s_H = sm + N2;  // H, or HL
+  C* s_HR = sm + 2 * N2;
+  C* s_tab = sm + 3 * N2;
+  C* s_x = s_tab + static_cast<size_t>(per) * nn * N2;  // v_u (DIAG) or T (dense)
+  const C* in = static_cast<const C*>(pick(A.in, cur));
   const int e = threadIdx.x;
   const bool ok = e < N2;
   const int i = ok ? e / N : 0, j = ok ? e - (e / N) * N : 0;
-  const C* in = static_cast<const C*>(pick(A.in, cur));
   PreT<KP> pv;
   prefetch<F64, KP>(pv, A, cur, e, ok);
-  if (ok) {
-    s_rho[e] = in[e];
-    s_H[e] = conv<R>(d.H[e]);
+  for (int x = threadIdx.x; x < N2; x += blockDim.x) {
+    s_rho[x] = in[x];
+    s_H[x] = conv<R>(DIAG ? d.rf_HL[x] : d.H[x]);
+    if (DIAG) s_HR[x] = conv<R>(d.rf_HR[x]);
   }
+  for (int x = threadIdx.x; x < nn * N2; x += blockDim.x) {
+    const int u = x / N2, y = x - u * N2;
+    C* t = s_tab + static_cast<size_t>(u) * per * N2;
+    t[y] = conv<R>(d.rf_L[x]);
+    t[N2 + y] = conv<R>(d.rf_Ld[x]);
+    if (!DIAG) t[2 * N2 + y] = conv<R>(d.V[x]);
+  }
+  if (DIAG)
+    for (int x = threadIdx.x; x < nn * N; x += blockDim.x) s_x[x] = conv<R>(d.Vd[x]);
   __syncthreads();
   C D{0, 0};
-  if (ok) {
-    C c{0, 0};
-    for (int k = 0; k < N; ++k) {
-      cmac(c, s_H[i * N + k], s_rho[k * N + j]);
-      cmac(c, s_rho[i * N + k], C{-s_H[k * N + j].re, -s_H[k * N + j].im});
-    }
-    D = times_i(c);
-  }
-  for (int u = 0; u < d.n_noise; ++u) {
-    const long long b = static_cast<long long>(u) * N2;
-    if (ok) {
-      s_L[e] = conv<R>(d.rf_L[b + e]);
-      s_Ld[e] = conv<R>(d.rf_Ld[b + e]);
-      s_V[e] = conv<R>(d.V[b + e]);
-    }
-    __syncthreads();
-    if (ok) {
-      C t{0, 0};
-      for (int k = 0; k < N; ++k) {
-        cmac(t, s_L[i * N + k], s_rho[k * N + j]);
-        cmac(t, s_rho[i * N + k], C{-s_Ld[k * N + j].re, -s_Ld[k * N + j].im});
-      }
-      s_T[e] = t;
-    }
-    __syncthreads();
+  if (DIAG) {
     if (ok) {
       for (int k = 0; k < N; ++k) {
-        cmac(D, s_V[i * N + k], s_T[k * N + j]);
-        cmac(D, s_T[i * N + k], C{-s_V[k * N + j].re, -s_V[k * N + j].im});
+        cmac(D, s_H[i * N + k], s_rho[k * N + j]);
+        cmac(D, s_rho[i * N + k], s_HR[k * N + j]);
+      }
+      for (int u = 0; u < nn; ++u) {
+        const C* L = s_tab + static_cast<size_t>(u) * 2 * N2;
+        const C* Ld = L + N2;
+        C x{0, 0}, y{0, 0};
+        for (int k = 0; k < N; ++k) {
+          cmac(x, s_rho[i * N + k], Ld[k * N + j]);
+          cmac(y, L[i * N + k], s_rho[k * N + j]);
+        }
+        const C vi = s_x[u * N + i], vj = s_x[u * N + j];
+        cmac(D, C{-vi.re, -vi.im}, x);
+        cmac(D, C{-vj.re, -vj.im}, y);
       }
     }
-    __syncthreads();
+  } else {
+    if (ok) {
+      C c{0, 0};
+      for (int k = 0; k < N; ++k) {
+        cmac(c, s_H[i * N + k], s_rho[k * N + j]);
+        cmac(c, s_rho[i * N + k], C{-s_H[k * N + j].re, -s_H[k * N + j].im});
+      }
+      D = times_i(c);
+    }
+    for (int u = 0; u < nn; ++u) {
+      const C* L = s_tab + static_cast<size_t>(u) * 3 * N2;
+      const C* Ld = L + N2;
+      const C* V = L + 2 * N2;
+      if (ok) {
+        C t{0, 0};
+        for (int k = 0; k < N; ++k) {
+          cmac(t, L[i * N + k], s_rho[k * N + j]);
+          cmac(t, s_rho[i * N + k], C{-Ld[k * N + j].re, -Ld[k * N + j].im});
+        }
+        s_x[e] = t;
+      }
+      __syncthreads();
+      if (ok) {
+        for (int k = 0; k < N; ++k) {
+          cmac(D, V[i * N + k], s_x[k * N + j]);
+          cmac(D, s_x[i * N + k], C{-V[k * N + j].re, -V[k * N + j].im});
+        }
+      }
+      __syncthreads();
+    }
   }
   double err = 0.0;
   if (ok) err = epilogue<F64, R, KP>(pv, A, cur, e, D, pg.alpha);
   if (A.do_err) write_partial(A, err);
 }
 
-// n_level > 16, first pass: T_u = Lambda_u rho - rho Lambda_u^+ for every
+// Larger problems, first pass: T_u = Lambda_u rho - rho Lambda_u^+ for every
 // noise, one lane per (u, element); operands come through L1/L2.
 template <typename R>
 __global__ void __launch_bounds__(256) redfield_t_kernel(const FormDev d, const StageArgs A) {
@@ -1226,7 +1267,7 @@ __global__ void __launch_bounds__(256) redfield_t_kernel(const FormDev d, const 
   }
 }
 
-// n_level > 16, second pass: D = i[H, rho] + sum_u [V_u, T_u] (diagonal V:
+// Larger problems, second pass: D = i[H, rho] + sum_u [V_u, T_u] (diagonal V:
 // (v_i - v_j) T_u[i][j]) and the stage epilogue, one lane per element.
 template <typename R, bool F64, int KP>
 __global__ void __launch_bounds__(256) redfield_d_kernel(const FormDev d, const StageArgs A) {
@@ -1252,8 +1293,7 @@ __global__ void __launch_bounds__(256) redfield_d_kernel(const FormDev d, const 
     for (int u = 0; u < d.n_noise; ++u) {
       const long long b = static_cast<long long>(u) * N2;
       if (d.diag_v) {
-        const C vij = conv<R>(d.V[b + i * N + i]) - conv<R>(d.V[b + j * N + j]);
-        cmac(D, vij, T[b + e]);
+        cmac(D, conv<R>(d.Vd[u * N + i]) - conv<R>(d.Vd[u * N + j]), T[b + e]);
       } else {
         for (int k = 0; k < N; ++k) {
           cmac(D, conv<R>(d.V[b + i * N + k]), T[b + k * N + j]);
@@ -1476,12 +1516,18 @@ int launch_redfield(const FormDev& d, const StageArgs& A, hipStream_t st) {
       hipLaunchKernelGGL((csr_wave_kernel<R, F64, kMaxPre>), dim3(grid), dim3(256), 0, st, d, A);
     return grid;
   }
-  if (d.N <= 16) {
-    if (small)
-      hipLaunchKernelGGL((redfield_small_kernel<R, F64, 4>), dim3(1), dim3(256), 0, st, d, A);
-    else
-      hipLaunchKernelGGL((redfield_small_kernel<R, F64, kMaxPre>), dim3(1), dim3(256), 0, st, d,
-                         A);
+  const bool diag = d.rf_HL != nullptr;
+  const size_t lds = redfield_lds_elems(d.N2, d.N, d.n_noise, diag) * sizeof(cx<R>);
+  if (d.N <= 16 && lds <= kRedfieldLds) {
+#define HAMD_RF(DG, KPV)                                                                    \
+  hipLaunchKernelGGL((redfield_small_kernel<R, F64, KPV, DG>), dim3(1), dim3(256), lds, st, d, \
+                     A)
+    if (diag) {
+      if (small) HAMD_RF(true, 4); else HAMD_RF(true, kMaxPre);
+    } else {
+      if (small) HAMD_RF(false, 4); else HAMD_RF(false, kMaxPre);
+    }
+#undef HAMD_RF
     return 1;
   }
   if (d.rf_T == nullptr) throw HeomError{HEOM_ERR_STATE, "redfield_hilb: scratch missing"};

@@ -99,7 +99,7 @@ class Qme {
   // and Lambda_u (logical N x N row-major, before any col_major transform)
   std::vector<double> rf_eig;
   std::vector<std::vector<z128>> rf_lambda;
-  DevBuf d_rf_L, d_rf_Ld, d_rf_T;
+  DevBuf d_rf_L, d_rf_Ld, d_rf_T, d_rf_HL, d_rf_HR;
   // learned truncation closure (mlp.h), null when none is bound
   std::unique_ptr<Closure> closure;
   void clear_closure();

@@ -9,6 +9,7 @@
 // D(rho) = i[H, rho] + sum_u [V_u, Lambda_u rho - rho Lambda_u^+]
 // (redfield_hilb.h:81-88 with drho = beta*drho - alpha*D), and the
 // Liouville form R = L - sum_u Phi_u Theta_u (redfield_liou.h:61-87).
+#include "kernels.h"
 #include "qme.h"
 #include "supop.h"
 
@@ -273,8 +274,35 @@ void Qme::set_param_redfield() {
   d_rf_Ld.upload(Ld);
   dev.rf_L = d_rf_L.as<z128>();
   dev.rf_Ld = d_rf_Ld.as<z128>();
-  // the two-pass Hilbert kernel keeps Lambda_u rho - rho Lambda_u^+ here
-  if (kind == HEOM_REDFIELD_HILB && N > 16) {
+  dev.rf_HL = dev.rf_HR = nullptr;
+  if (kind == HEOM_REDFIELD_HILB && diag_v) {
+    // [V, Lambda rho - rho Lambda^+] = V Lambda rho + rho Lambda^+ V
+    //                                  - V rho Lambda^+ - Lambda rho V
+    // the first two terms of every noise fold into HL / HR
+    std::vector<z128> HL(N2), HR(N2);
+    for (int e = 0; e < N2; ++e) {
+      HL[e] = zmul(kI, Hw[e]);
+      HR[e] = zscale(-1.0, HL[e]);
+    }
+    for (int u = 0; u < nn; ++u) {
+      const std::vector<z128>& Vu = nz[u].V;
+      const z128* Lu = L.data() + static_cast<size_t>(u) * N2;
+      const z128* Ldu = Ld.data() + static_cast<size_t>(u) * N2;
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          HL[i * N + j] = HL[i * N + j] + zmul(Vu[i * N + i], Lu[i * N + j]);
+          HR[i * N + j] = HR[i * N + j] + zmul(Ldu[i * N + j], Vu[j * N + j]);
+        }
+    }
+    d_rf_HL.upload(HL);
+    d_rf_HR.upload(HR);
+    dev.rf_HL = d_rf_HL.as<z128>();
+    dev.rf_HR = d_rf_HR.as<z128>();
+  }
+  // the two-pass Hilbert kernels keep Lambda_u rho - rho Lambda_u^+ here
+  if (kind == HEOM_REDFIELD_HILB &&
+      (N > 16 || redfield_lds_elems(N2, N, nn, diag_v) * (lo() ? sizeof(z64) : sizeof(z128)) >
+                     kRedfieldLds)) {
     d_rf_T.alloc(static_cast<size_t>(nn) * N2 * sizeof(z128));
     dev.rf_T = d_rf_T.p;
   } else {

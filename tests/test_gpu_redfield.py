@@ -1,9 +1,10 @@
 """Redfield QMEs on the GPU (redfield_hilb.h, redfield_liou.h) against the oracle.
 
 Lambda_u from the library's set-up (its own Hermitian eigensolver and LU
-solves) must equal the oracle's to 1e-12; calc_diff of both forms (the one-
-workgroup kernel for n_level <= 16, the two-pass kernels above, the
-wavefront-per-row superoperator product of the Liouville form) and RK4 / rkdp /
+solves) must equal the oracle's to 1e-12; calc_diff of both forms (Hilbert
+form: one workgroup with every table in LDS when n_level <= 16 and they fit,
+for diagonal and dense V, else two passes through a scratch block; Liouville
+form: the wavefront-per-row superoperator product) and RK4 / rkdp /
 lsrk4 solves must meet the FP64 bar of 1e-10 (relative to the magnitude of
 the result where it exceeds 1).  complex64 states its error.  col_major
 storage is checked as the transpose of the row-major result.
@@ -38,6 +39,26 @@ def pair(form, m, dtype="complex128", order=H.row_major):
 
 def rel(got, ref):
     return np.abs(got - ref).max() / max(1.0, np.abs(ref).max())
+
+
+def physical_model(N, dense_v, seed):
+    """Random H (spectral radius 1) and two Drude baths (lambda 0.2, gamma 1, T 1,
+    one Pade mode) coupled through random Hermitian (or diagonal) V of norm 1;
+    random complex bath parameters can make the Redfield generator grow without
+    bound, a physical bath does not."""
+    from libheom_amd import bath
+    rng = np.random.default_rng(seed)
+
+    def herm():
+        X = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+        X = 0.5 * (X + X.conj().T)
+        return X / np.abs(np.linalg.eigvalsh(X)).max()
+
+    nu, c, delta = bath.drude_modes(0.2, 1.0, 1.0, 1, "pade")
+    noises = [bath.noise_params(herm() if dense_v else
+                                np.diag(rng.normal(size=N)).astype(np.complex128), nu, c, delta)
+              for _ in range(2)]
+    return {"N": N, "H": herm(), "noises": noises, "max_depth": 0}
 
 
 def spin_boson_redfield(**kw):
@@ -131,13 +152,15 @@ def test_rkdp_chain7_same_steps(form):
     assert abs(np.trace(rho.reshape(7, 7)) - 1.0) <= 1e-10
 
 
-def test_two_pass_kernel_solve():
-    """n_level 32 (two-pass kernels), RK4 dt=0.005 to t=1."""
-    m = configs.random_model(32, 2, 2, 0, seed=32, dense_v=False)
-    m["H"] = m["H"] * 0.2
+@pytest.mark.parametrize("N,dense_v", [(4, True), (32, True), (32, False)])
+def test_rk4_solve_each_hilbert_kernel(N, dense_v):
+    """RK4 dt=0.005 to t=1 through the Hilbert-form paths not covered by the
+    spin-boson and chain solves (one workgroup, diagonal V): one workgroup with
+    dense V, two passes with dense and with diagonal V."""
+    m = physical_model(N, dense_v, seed=N)
     t = np.linspace(0.0, 1.0, 5)
     g, o = pair("hilb", m)
-    X = np.random.default_rng(1).normal(size=(32, 32))
+    X = np.random.default_rng(1).normal(size=(N, N))
     rho0 = (X @ X.T).astype(np.complex128)
     rho0 = (rho0 / np.trace(rho0)).reshape(-1)
     s = H.qme_solver(eng(), g, H.rk4())

@@ -65,6 +65,11 @@ typedef struct heom_options {
   int device;     /* HIP device ordinal */
   int matrix_cores; /* 1 (default): heom_hilb with n_level 32 or 64 and diagonal
                        V computes the commutator on MFMA; 0: vector ALU kernels */
+  int renorm_rho0;  /* 1: after every RK step / Dormand-Prince attempt, replace rho_0
+                       (rho_S) by (rho_0 + rho_0^+)/2 scaled to the trace it had when
+                       the solve started, in FP64 on the device (n_level <= 64, not
+                       for batches).  0 (default, parity mode): off.  No counterpart
+                       in the reference; for the complex64 variants. */
 } heom_options;
 
 typedef struct heom_stats {

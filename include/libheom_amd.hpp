@@ -154,6 +154,7 @@ class qme_base {
     opt.parity_q1 = parity_q1;
     opt.sqrt_norm = sqrt_normalization;
     opt.matrix_cores = matrix_cores;
+    opt.renorm_rho0 = renorm_rho0;
     const int dt = std::is_same<dtype, complex64>::value ? HEOM_C64 : HEOM_C128;
     check(heom_create(kind_, dt, n_level, max_depth_, &opt, &h_));
     check(heom_set_H(h_, H.dense(n_level, n_level).data()));
@@ -194,6 +195,7 @@ class qme_base {
   int parity_q1 = 1;
   int sqrt_normalization = 0;
   int matrix_cores = 1;  // MFMA commutator where provided (n_level 32 / 64, diagonal V)
+  int renorm_rho0 = 0;   // FP64 trace / hermiticity projection of rho_0 after every step
 
  protected:
   static int filter_trampoline(const int* index, int n_dim, int depth, void* user) {

@@ -59,7 +59,8 @@ class _heom:
     _kind = None
 
     def __init__(self, max_depth, n_inner_threads=1, n_outer_threads=1, dtype="complex128",
-                 order=row_major, parity_q1=True, sqrt_normalization=False, matrix_cores=True):
+                 order=row_major, parity_q1=True, sqrt_normalization=False, matrix_cores=True,
+                 renorm_rho0=False):
         # heom.h:44-50; thread counts are accepted for API parity, the device replaces them
         self.max_depth = int(max_depth)
         self.n_inner_threads = n_inner_threads
@@ -73,6 +74,8 @@ class _heom:
             raise ValueError("dtype must be complex128, complex64 or complex64_bf16x3")
         self.dtype = np.dtype(dtype)
         self.matrix_cores = matrix_cores
+        # project rho_0 on Hermitian matrices of the initial trace after every step
+        self.renorm_rho0 = renorm_rho0
         self.order = order
         self.parity_q1 = parity_q1
         self.sqrt_normalization = sqrt_normalization
@@ -115,7 +118,7 @@ class _heom:
             self._h = None
         opt = _capi.heom_options(int(self.parity_q1), int(self.sqrt_normalization),
                                  0 if self.order == row_major else 1, dev,
-                                 int(bool(self.matrix_cores)))
+                                 int(bool(self.matrix_cores)), int(bool(self.renorm_rho0)))
         h = ctypes.c_void_p()
         dt = _capi.HEOM_C64 if self.dtype == np.complex64 else _capi.HEOM_C128
         if self.bf16x3:
@@ -249,8 +252,8 @@ class _redfield(_heom):
     diagonalises H and builds Lambda_u from the bath correlations.  The state is
     one N x N density matrix (main_size = N^2)."""
 
-    def __init__(self, dtype="complex128", order=row_major):
-        super().__init__(0, dtype=dtype, order=order)
+    def __init__(self, dtype="complex128", order=row_major, renorm_rho0=False):
+        super().__init__(0, dtype=dtype, order=order, renorm_rho0=renorm_rho0)
 
     def Lambda(self, u):
         """Lambda_u after set_param (redfield.h:30, 166), N x N."""

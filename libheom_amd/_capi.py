@@ -51,7 +51,7 @@ class HeomError(RuntimeError):
 class heom_options(ctypes.Structure):
     _fields_ = [("parity_q1", ctypes.c_int), ("sqrt_norm", ctypes.c_int),
                 ("order", ctypes.c_int), ("device", ctypes.c_int),
-                ("matrix_cores", ctypes.c_int)]
+                ("matrix_cores", ctypes.c_int), ("renorm_rho0", ctypes.c_int)]
 
 
 class heom_stats(ctypes.Structure):

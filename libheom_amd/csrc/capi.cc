@@ -99,6 +99,7 @@ int heom_default_options(heom_options* opt) {
   opt->order = 0;
   opt->device = 0;
   opt->matrix_cores = 1;
+  opt->renorm_rho0 = 0;
   return HEOM_OK;
 }
 

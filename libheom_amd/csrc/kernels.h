@@ -39,6 +39,12 @@ void launch_closure_gather(const FormDev& d, const StageArgs& A, float* x, int a
                            hipStream_t stream);
 void launch_gather_rho_s(const FormDev& d, void* dst, const void* src, hipStream_t stream);
 
+// Tr rho_0 (real part) of the FP64 state y into *tr0
+void launch_rho0_trace(const void* y, int N, double* tr0, hipStream_t stream);
+// rho_0 <- (rho_0 + rho_0^+)/2 * tr0 / Tr rho_0 in the FP64 state of parity
+// ctrl->cur (y0 / y1; parity 0 without ctrl), and its FP32 copy f0 / f1 if given
+void launch_rho0_renorm(void* y0, void* y1, void* f0, void* f1, const Ctrl* ctrl, int N,
+                        const double* tr0, hipStream_t stream);
 // dst (FP32 complex) <- src (FP64 complex), n elements
 void launch_narrow(void* dst, const void* src, long long n, hipStream_t stream);
 // dst <- src for n complex elements of the given slot type

@@ -1337,6 +1337,56 @@ __global__ void __launch_bounds__(256) csr_wave_kernel(const FormDev d, const St
   if (A.do_err) write_partial(A, err_acc);
 }
 
+// rho_0 renormalisation (optional): Tr rho_0 of the initial state, then after
+// every step rho_0 <- (rho_0 + rho_0^+)/2 * tr0 / Tr, in FP64.  One workgroup of
+// 1024 lanes, up to four elements each (n_level <= 64); every element and its
+// transpose are read before any lane writes.  y0/y1 (and the FP32 copies f0/f1,
+// may be null) are the two parities of the state; ctrl == null: parity 0.
+__global__ void __launch_bounds__(1024) rho0_trace_kernel(const z128* y, int N, double* tr0) {
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < N; ++i) t += y[i * N + i].re;
+    *tr0 = t;
+  }
+}
+
+__global__ void __launch_bounds__(1024) rho0_renorm_kernel(z128* y0, z128* y1, z64* f0, z64* f1,
+                                                          const Ctrl* ctrl, int N,
+                                                          const double* tr0) {
+  __shared__ double s_factor;
+  const int cur = ctrl ? ctrl->cur : 0;
+  z128* y = cur ? y1 : y0;
+  z64* f = cur ? f1 : f0;
+  const int N2 = N * N;
+  z128 v[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = threadIdx.x + r * 1024;
+    if (e < N2) {
+      const int i = e / N, j = e - i * N;
+      const z128 a = y[e], b = y[j * N + i];
+      v[r] = z128{0.5 * (a.re + b.re), 0.5 * (a.im - b.im)};
+    }
+  }
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < N; ++i) t += y[i * N + i].re;
+    const double target = *tr0;
+    s_factor = (t != 0.0 && isfinite(t) && isfinite(target)) ? target / t : 1.0;
+  }
+  __syncthreads();
+  const double g = s_factor;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = threadIdx.x + r * 1024;
+    if (e < N2) {
+      const z128 w{g * v[r].re, g * v[r].im};
+      y[e] = w;
+      if (f) f[e] = z64{static_cast<float>(w.re), static_cast<float>(w.im)};
+    }
+  }
+}
+
 __global__ void ctrl_init_kernel(Ctrl* c, double t0, double t_bound, double dt) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (dt < 0.0) dt = c->dt;  // keep dt_save (adaptive_step_size_solver.h:39-41,55)
@@ -1610,6 +1660,21 @@ void launch_closure_gather(const FormDev& d, const StageArgs& A, float* x, int a
 void launch_gather_rho_s(const FormDev& d, void* dst, const void* src, hipStream_t stream) {
   hipLaunchKernelGGL(gather_rho_s_kernel, dim3(d.n_inst), dim3(64), 0, stream,
                      static_cast<z128*>(dst), static_cast<const z128*>(src), d.inst);
+  HAMD_HIP(hipGetLastError());
+}
+
+void launch_rho0_trace(const void* y, int N, double* tr0, hipStream_t stream) {
+  hipLaunchKernelGGL(rho0_trace_kernel, dim3(1), dim3(64), 0, stream, static_cast<const z128*>(y),
+                     N, tr0);
+  HAMD_HIP(hipGetLastError());
+}
+
+void launch_rho0_renorm(void* y0, void* y1, void* f0, void* f1, const Ctrl* ctrl, int N,
+                        const double* tr0, hipStream_t stream) {
+  if (N > 64) throw HeomError{HEOM_ERR_UNSUPPORTED, "renorm_rho0: n_level > 64"};
+  hipLaunchKernelGGL(rho0_renorm_kernel, dim3(1), dim3(1024), 0, stream, static_cast<z128*>(y0),
+                     static_cast<z128*>(y1), static_cast<z64*>(f0), static_cast<z64*>(f1), ctrl,
+                     N, tr0);
   HAMD_HIP(hipGetLastError());
 }
 

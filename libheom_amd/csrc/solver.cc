@@ -91,6 +91,12 @@ Solver::Solver(Qme* q_, int kind_) : q(q_), kind(kind_) {
   } else {
     for (int k = 0; k < 6; ++k) mk(SL_K1 + k, sinb, lo);
   }
+  if (q->opt.renorm_rho0) {
+    if (q->kind == kBatch)
+      throw HeomError{HEOM_ERR_UNSUPPORTED, "renorm_rho0 is not provided for batches"};
+    if (q->N > 64) throw HeomError{HEOM_ERR_UNSUPPORTED, "renorm_rho0: n_level > 64"};
+    d_tr0.alloc(sizeof(double));
+  }
   d_ctrl.alloc(sizeof(Ctrl));
   HAMD_HIP(hipMemset(d_ctrl.p, 0, sizeof(Ctrl)));
   d_partials.alloc(sizeof(double) * kMaxGrid);
@@ -248,6 +254,15 @@ void Solver::build_stages(double atol, double rtol) {
   }
 }
 
+// rho_0 projection after a step / attempt (heom_options.renorm_rho0)
+void Solver::renorm(bool with_ctrl) {
+  const bool lo = in_bytes == 4;
+  launch_rho0_renorm(slot_ptr(SL_Y, 0), slot_ptr(SL_Y, with_ctrl ? 1 : 0),
+                     lo ? slot_ptr(SL_Y32, 0) : nullptr,
+                     lo ? slot_ptr(SL_Y32, with_ctrl ? 1 : 0) : nullptr,
+                     with_ctrl ? d_ctrl.as<Ctrl>() : nullptr, q->N, d_tr0.as<double>(), q->stream);
+}
+
 void Solver::launch_stage(int s, double dt) {
   StageArgs a = stages[s];
   a.a_re = dt;
@@ -329,12 +344,16 @@ void Solver::rk4_interval(double t0, double t1, double dt) {
     }
     replay(g_rk4, n_full, [&] {
       for (int s = 0; s < 4; ++s) launch_stage(s, dt);
+      if (q->opt.renorm_rho0) renorm(false);
     });
     done = n_full;
   }
-  for (; done < n_full; ++done)
+  for (; done < n_full; ++done) {
     for (int s = 0; s < 4; ++s) launch_stage(s, dt);
+    if (q->opt.renorm_rho0) renorm(false);
+  }
   for (int s = 0; s < 4; ++s) launch_stage(s, dt_1);  // the last (shortened) step
+  if (q->opt.renorm_rho0) renorm(false);
   stats.n_rhs += 4 * (n_full + 1);
   stats.n_attempts += n_full + 1;
   stats.n_accepted += n_full + 1;
@@ -345,8 +364,11 @@ void Solver::rk4_interval(double t0, double t1, double dt) {
 int Solver::rkdp_interval(double t0, double t1, double dt_first, int& batch) {
   launch_ctrl_init(d_ctrl.as<Ctrl>(), t0, t1, dt_first, q->stream);
   long long a0 = h_ctrl->attempts;
+  // a rejected attempt leaves the projected state unchanged, so projecting after
+  // every attempt equals projecting after every accepted step (up to rounding)
   const auto attempt = [&] {
     for (int s = 0; s < 7; ++s) launch_stage(s, 0.0);
+    if (q->opt.renorm_rho0) renorm(true);
   };
   while (true) {
     if (use_graphs) {
@@ -385,6 +407,7 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
   // the caller's state enters the FP64 state slot
   HAMD_HIP(hipMemcpyAsync(P[SL_Y].p, rho_dev, bytes, hipMemcpyDeviceToDevice, st));
   if (in_bytes == 4) launch_narrow(P[SL_Y32].p, P[SL_Y].p, ms, st);
+  if (q->opt.renorm_rho0) launch_rho0_trace(P[SL_Y].p, q->N, d_tr0.as<double>(), st);
   int gave_up = 0;
   if (kind != HEOM_RKDP) {  // fixed step: rk4, lsrk4
     for (int i = 0; i < n_t - 1; ++i) {

@@ -76,6 +76,8 @@ class Solver {
   void* sp[kSlots] = {};
   int slo[kSlots] = {};  // 1: complex64 buffer
   DevBuf d_ctrl, d_partials;
+  DevBuf d_tr0;  // Tr rho_0 at the start of the solve (renorm_rho0)
+  void renorm(bool with_ctrl);
   Ctrl* h_ctrl = nullptr;  // pinned mirror
   std::vector<StageArgs> stages;
 

@@ -220,6 +220,30 @@ struct FormDev {
   const BatchItem* items;
 };
 
+// Resident fixed-step integrator (hierarchies of at most kResidentMax
+// elements): the stage program of a fixed-step scheme with the solver's slots
+// renumbered to LDS buffers 0..n_buf-1 (buffer 0 holds the state), run for a
+// whole output interval by one workgroup.
+constexpr int kResidentMax = 1024;
+constexpr size_t kResidentLds = 156 * 1024;  // dynamic LDS (the stage program is static)
+constexpr int kResidentPre = 4;
+struct ResidentStage {
+  int in;  // buffer read by the operator
+  int npre;
+  int pre[kResidentPre];
+  int ncombo;
+  double cc[2][kResidentPre];
+  double ck[2];
+  int dst[2];
+};
+struct ResidentArgs {
+  int n_stages, n_buf;
+  ResidentStage st[4];
+  z128* y;          // the FP64 state in HBM (read at the start, written at the end)
+  long long n_full; // steps of dt, then one step of dt_last
+  double dt, dt_last;
+};
+
 // ---------------------------------------------------------------------------
 // host-side error helpers
 // ---------------------------------------------------------------------------

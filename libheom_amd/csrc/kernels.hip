@@ -1387,6 +1387,104 @@ __global__ void __launch_bounds__(1024) rho0_renorm_kernel(z128* y0, z128* y1, z
   }
 }
 
+// ---------------------------------------------------------------------------
+// Resident fixed-step integrator: heom_hilb with diagonal V, FP64, at most
+// kResidentMax elements.  One workgroup keeps the state and every stage buffer
+// in LDS and runs all steps of an output interval; one lane per element.  The
+// per-element link coefficients (neighbour element, c = cm*i*phi + ac*psi +
+// dir, the same expression the fast kernel forms per stage) and Gamma_n +
+// delta term are computed once into registers.  Stage arithmetic follows the
+// epilogue of the stage kernels (k = -dt*D, sum of c_s*operand_s then + ck*k).
+// ---------------------------------------------------------------------------
+template <int MAXL>
+__global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const ResidentArgs a) {
+  extern __shared__ __align__(16) unsigned char res_lds[];
+  __shared__ ResidentStage s_st[4];
+  z128* buf = reinterpret_cast<z128*>(res_lds);
+  if (threadIdx.x == 0) {  // constant indices: the argument block stays in the constant space
+    s_st[0] = a.st[0];
+    s_st[1] = a.st[1];
+    s_st[2] = a.st[2];
+    s_st[3] = a.st[3];
+  }
+  const int ms = static_cast<int>(d.main_size);
+  const int N = d.N, N2 = d.N2;
+  z128* s_H = buf + static_cast<size_t>(a.n_buf) * ms;
+  const int e = threadIdx.x;
+  const bool ok = e < ms;
+  for (int x = e; x < a.n_buf * ms; x += blockDim.x)
+    buf[x] = x < ms ? a.y[x] : z128{0.0, 0.0};
+  for (int x = e; x < N2; x += blockDim.x) s_H[x] = d.H[x];
+  const int g = ok ? e / N2 : 0;
+  const int ee = ok ? e - g * N2 : 0;
+  const int i = ee / N, j = ee - (ee / N) * N;
+  const int l0 = d.link_ptr[g], l1 = d.link_ptr[g + 1];
+  const int nl = ok ? l1 - l0 : 0;
+  const z128 diag = d.gam[g] + d.dterm[ee];
+  int nbr[MAXL];
+  z128 cf[MAXL];
+#pragma unroll
+  for (int l = 0; l < MAXL; ++l) {
+    nbr[l] = 0;
+    cf[l] = z128{0.0, 0.0};
+    if (l < nl) {
+      const LinkFlat q = d.flat[l0 + l];
+      nbr[l] = q.nbr * N2 + ee;
+      const long long b = static_cast<long long>(q.u) * N2 + ee;
+      if (d.real_v) {
+        const z128 p = d.cmac_tab[b];
+        cf[l].re = p.re * q.iphi.re + p.im * q.psi.re + q.dir.re;
+        cf[l].im = p.re * q.iphi.im + p.im * q.psi.im + q.dir.im;
+      } else {
+        cf[l] = d.cmac_tab[2 * b] * q.iphi + d.cmac_tab[2 * b + 1] * q.psi + q.dir;
+      }
+    }
+  }
+  __syncthreads();
+  const long long n_steps = a.n_full + 1;
+  for (long long step = 0; step < n_steps; ++step) {
+    const double dt = step < a.n_full ? a.dt : a.dt_last;
+    for (int s = 0; s < a.n_stages; ++s) {
+      const ResidentStage& st = s_st[s];
+      if (ok) {
+        const z128* X = buf + static_cast<size_t>(st.in) * ms;
+        const z128* Xg = X + static_cast<size_t>(g) * N2;
+        z128 acc{0.0, 0.0};
+        for (int k = 0; k < N; ++k) {
+          cmac(acc, s_H[i * N + k], Xg[k * N + j]);
+          const z128 t = Xg[i * N + k];
+          cmac(acc, z128{-t.re, -t.im}, s_H[k * N + j]);
+        }
+        z128 D = times_i(acc);
+        cmac(D, diag, X[e]);
+#pragma unroll
+        for (int l = 0; l < MAXL; ++l)
+          if (l < nl) cmac(D, cf[l], X[nbr[l]]);
+        const z128 k{-(dt * D.re - 0.0 * D.im), -(dt * D.im + 0.0 * D.re)};
+        z128 pv[kResidentPre];
+#pragma unroll
+        for (int p = 0; p < kResidentPre; ++p)
+          pv[p] = p < st.npre ? buf[static_cast<size_t>(st.pre[p]) * ms + e] : z128{0.0, 0.0};
+        for (int c = 0; c < st.ncombo; ++c) {
+          z128 v{0.0, 0.0};
+#pragma unroll
+          for (int p = 0; p < kResidentPre; ++p) {
+            if (p < st.npre) {
+              v.re = fma(st.cc[c][p], pv[p].re, v.re);
+              v.im = fma(st.cc[c][p], pv[p].im, v.im);
+            }
+          }
+          v.re = fma(st.ck[c], k.re, v.re);
+          v.im = fma(st.ck[c], k.im, v.im);
+          buf[static_cast<size_t>(st.dst[c]) * ms + e] = v;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (ok) a.y[e] = buf[e];
+}
+
 __global__ void ctrl_init_kernel(Ctrl* c, double t0, double t_bound, double dt) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (dt < 0.0) dt = c->dt;  // keep dt_save (adaptive_step_size_solver.h:39-41,55)
@@ -1675,6 +1773,32 @@ void launch_rho0_renorm(void* y0, void* y1, void* f0, void* f1, const Ctrl* ctrl
   hipLaunchKernelGGL(rho0_renorm_kernel, dim3(1), dim3(1024), 0, stream, static_cast<z128*>(y0),
                      static_cast<z128*>(y1), static_cast<z64*>(f0), static_cast<z64*>(f1), ctrl,
                      N, tr0);
+  HAMD_HIP(hipGetLastError());
+}
+
+bool resident_ok(const FormDev& d, int max_links) {
+  return d.kind == HEOM_HILB && d.diag_v && d.main_size <= kResidentMax && max_links <= 8 &&
+         d.clos == nullptr;
+}
+
+void launch_resident(const FormDev& d, int max_links, const ResidentArgs& a, hipStream_t stream) {
+  const int ms = static_cast<int>(d.main_size);
+  if (!resident_ok(d, max_links) || a.n_stages < 1 || a.n_stages > 4 || a.n_buf < 1)
+    throw HeomError{HEOM_ERR_STATE, "resident integrator: unsupported problem"};
+  const size_t lds = sizeof(z128) * (static_cast<size_t>(a.n_buf) * ms + d.N2);
+  if (lds > kResidentLds) throw HeomError{HEOM_ERR_STATE, "resident integrator: LDS exceeded"};
+  const int bs = ((ms + 63) / 64) * 64;
+#define HAMD_RES(L)                                                                   \
+  do {                                                                                \
+    static bool once = (allow_lds(resident_kernel<L>, kResidentLds), true);           \
+    (void)once;                                                                       \
+    hipLaunchKernelGGL(resident_kernel<L>, dim3(1), dim3(bs), lds, stream, d, a);      \
+  } while (0)
+  if (max_links <= 4)
+    HAMD_RES(4);
+  else
+    HAMD_RES(8);
+#undef HAMD_RES
   HAMD_HIP(hipGetLastError());
 }
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <set>
 
 #include "kernels.h"
@@ -254,6 +255,47 @@ void Solver::build_stages(double atol, double rtol) {
   }
 }
 
+// The fixed-step stage program in LDS-buffer form for the resident integrator
+// (slot SL_Y -> buffer 0, the other slots in order of first use).
+void Solver::build_resident() {
+  res = ResidentArgs{};
+  if (!use_graphs || kind == HEOM_RKDP || in_bytes != 8 || q->opt.renorm_rho0 ||
+      !resident_ok(q->dev, q->dev.max_links))
+    return;
+  std::map<int, int> bmap{{SL_Y, 0}};
+  auto b = [&](int slot) {
+    auto it = bmap.find(slot);
+    if (it != bmap.end()) return it->second;
+    const int id = static_cast<int>(bmap.size());
+    bmap[slot] = id;
+    return id;
+  };
+  ResidentArgs r{};
+  r.n_stages = static_cast<int>(specs.size());
+  if (r.n_stages > 4) return;
+  for (int s = 0; s < r.n_stages; ++s) {
+    const StageSpec& sp = specs[s];
+    if (static_cast<int>(sp.pre.size()) > kResidentPre || sp.ncombo > 2 || sp.k_slot >= 0 ||
+        sp.do_err)
+      return;
+    ResidentStage& t = r.st[s];
+    t.in = b(sp.in_slot);
+    t.npre = static_cast<int>(sp.pre.size());
+    for (int p = 0; p < t.npre; ++p) t.pre[p] = b(sp.pre[p]);
+    t.ncombo = sp.ncombo;
+    for (int c = 0; c < sp.ncombo; ++c) {
+      for (int p = 0; p < t.npre; ++p) t.cc[c][p] = sp.cc[c][p];
+      t.ck[c] = sp.ck[c];
+      t.dst[c] = b(sp.dst[c]);
+      if (sp.dst2[c] >= 0) return;
+    }
+  }
+  r.n_buf = static_cast<int>(bmap.size());
+  const size_t lds = sizeof(z128) * (static_cast<size_t>(r.n_buf) * ms + q->N2);
+  if (lds > kResidentLds) return;
+  res = r;
+}
+
 // rho_0 projection after a step / attempt (heom_options.renorm_rho0)
 void Solver::renorm(bool with_ctrl) {
   const bool lo = in_bytes == 4;
@@ -336,6 +378,18 @@ void Solver::rk4_interval(double t0, double t1, double dt) {
     ++n_full;
     t += dt_1;
   }
+  if (res.n_stages > 0) {  // the whole interval in one launch
+    ResidentArgs a = res;
+    a.y = static_cast<z128*>(P[SL_Y].p);
+    a.n_full = n_full;
+    a.dt = dt;
+    a.dt_last = dt_1;
+    launch_resident(q->dev, q->dev.max_links, a, q->stream);
+    stats.n_rhs += 4 * (n_full + 1);
+    stats.n_attempts += n_full + 1;
+    stats.n_accepted += n_full + 1;
+    return;
+  }
   long long done = 0;
   if (use_graphs && n_full > 0) {
     if (g_rk4_dt != dt) {
@@ -401,6 +455,7 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
   }
   HAMD_HIP(hipSetDevice(q->opt.device));
   build_stages(atol, rtol);
+  build_resident();
   reset_graphs();  // stage arguments (tolerances, closure) may have changed
   const size_t bytes = sizeof(z128) * ms;
   hipStream_t st = q->stream;

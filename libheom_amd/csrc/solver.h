@@ -100,6 +100,10 @@ class Solver {
   void capture(GraphExec& g, F&& body);
   void reset_graphs();
   void rk4_interval(double t0, double t1, double dt);
+  // resident integrator (small heom_hilb problems): stage program built by
+  // build_stages when the problem qualifies, else n_stages == 0
+  ResidentArgs res{};
+  void build_resident();
   int rkdp_interval(double t0, double t1, double dt_first, int& batch);
 };
 

@@ -63,8 +63,9 @@ typedef struct heom_options {
   int sqrt_norm;  /* 1: LIBHEOM_SQRT_NORMALIZATION factors (heom_hilb.h:140-196) */
   int order;      /* 0: row_major (default), 1: col_major (matrix_base.h) */
   int device;     /* HIP device ordinal */
-  int matrix_cores; /* 1 (default): heom_hilb with n_level 32 or 64 and diagonal
-                       V computes the commutator on MFMA; 0: vector ALU kernels */
+  int matrix_cores; /* 1 (default): heom_hilb with 16 < n_level <= 64 and diagonal
+                       V computes the commutator on MFMA (blocks zero-padded to 32
+                       or 64 rows); 0: vector ALU kernels */
   int renorm_rho0;  /* 1: after every RK step / Dormand-Prince attempt, replace rho_0
                        (rho_S) by (rho_0 + rho_0^+)/2 scaled to the trace it had when
                        the solve started, in FP64 on the device (n_level <= 64, not

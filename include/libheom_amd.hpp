@@ -194,7 +194,7 @@ class qme_base {
   // reference quirk switches (parity defaults)
   int parity_q1 = 1;
   int sqrt_normalization = 0;
-  int matrix_cores = 1;  // MFMA commutator where provided (n_level 32 / 64, diagonal V)
+  int matrix_cores = 1;  // MFMA commutator where provided (16 < n_level <= 64, diagonal V)
   int renorm_rho0 = 0;   // FP64 trace / hermiticity projection of rho_0 after every step
 
  protected:

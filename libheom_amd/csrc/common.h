@@ -192,8 +192,9 @@ struct FormDev {
   const long long* R_ptr;
   const int* R_col;
   const z128* R_val;
-  // matrix-core path (heom_hilb, N in {32, 64}, diagonal V): H as separate
-  // real / imaginary planes in FP64 and FP32; mfma = 0 off, 1 on (operands in
+  // matrix-core path (heom_hilb, 16 < N <= 64, diagonal V): H as separate
+  // real / imaginary planes in FP64 and FP32, leading dimension 32 or 64 (the
+  // MFMA tile, zero-padded); mfma = 0 off, 1 on (operands in
   // the state precision), 2 split-bf16x3 products with FP32 accumulation
   int real_h;
   int mfma;

@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(kBigBS) hilb_big_kernel(const FormDev d, const
 }
 
 // ---------------------------------------------------------------------------
-// heom_hilb, N in {32, 64}, diagonal V: the commutator H*rho - rho*H on the
+// heom_hilb, 16 < N <= 64, diagonal V: the commutator H*rho - rho*H on the
 // matrix cores (heom_hilb.h:108-125 computes it with two gemm calls per ADO).
 // One ADO per 256-thread workgroup.  rho_n is staged in LDS as separate real
 // and imaginary planes with row stride N+1, so that the 16 rows of an MFMA
@@ -685,7 +685,7 @@ struct MfmaCfg {
   static constexpr int TPW = NB * NB / 4;         // output tiles per wave
   static constexpr int KS = N / 4;                // k steps of the 16x16x4 products
   static constexpr int KB = N / 32;               // k steps of the bf16 products
-  static_assert(N == 32 || N == 64, "MFMA path: N in {32, 64}");
+  static_assert(N == 32 || N == 64, "MFMA path: tiles of 32 or 64 rows");
 };
 
 // H*rho - rho*H for the wave's tiles, operands in the state precision R
@@ -784,7 +784,10 @@ __device__ __forceinline__ void mfma_commutator_bf16x3(const FormDev& d, const f
   }
 }
 
-// BF16X3: R = float, products via mfma_commutator_bf16x3
+// BF16X3: R = float, products via mfma_commutator_bf16x3.  N is the tile size
+// (32 or 64); the ADO block is n x n with n = d.N in (N/2, N] for N = 64, in
+// (16, 32] for N = 32, zero-padded in LDS (and H in its planes), so that the
+// padded rows and columns of the products are exactly zero.
 template <int N, typename R, bool REALH, bool F64, int KP, bool BF16X3>
 __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
@@ -796,6 +799,11 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
   R* s_im = s_re + N * LD;
   const Prologue pg = prologue(A);
   if (pg.skip) return;
+  const int n = d.N, n2 = d.N2;  // the ADO block (n <= N)
+  if (n < N) {  // the padding stays zero: only rows and columns < n are ever written
+    for (int x = threadIdx.x; x < 2 * N * LD; x += kMfmaBS) s_re[x] = R(0);
+    __syncthreads();
+  }
   const int cur = pg.cur;
   const C* in = static_cast<const C*>(pick(A.in, cur));
   const auto link_ptr = cst(d.link_ptr);
@@ -807,19 +815,24 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
   const int band = t0 / NB;
   double err_acc = 0.0;
   for (int g = blockIdx.x; g < d.n_hrchy; g += gridDim.x) {
-    const long long base = static_cast<long long>(g) * N2;
+    const long long base = static_cast<long long>(g) * n2;
     {
       // all loads of rho_n in flight before the LDS stores
       constexpr int PER = N2 / kMfmaBS;
       C v[PER];
 #pragma unroll
-      for (int c = 0; c < PER; ++c) v[c] = in[base + threadIdx.x + c * kMfmaBS];
+      for (int c = 0; c < PER; ++c) {
+        const int x = threadIdx.x + c * kMfmaBS;
+        v[c] = x < n2 ? in[base + x] : C{0, 0};
+      }
 #pragma unroll
       for (int c = 0; c < PER; ++c) {
         const int x = threadIdx.x + c * kMfmaBS;
-        const int i = x / N, j = x % N;
-        s_re[i * LD + j] = v[c].re;
-        s_im[i * LD + j] = v[c].im;
+        if (x < n2) {
+          const int i = x / n, j = x - (x / n) * n;
+          s_re[i * LD + j] = v[c].re;
+          s_im[i * LD + j] = v[c].im;
+        }
       }
     }
     __syncthreads();
@@ -843,29 +856,33 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = band * 16 + M::row(lk, r);
-        C D = times_i(C{cre[t][r], cim[t][r]});
-        const C rv{s_re[i * LD + col], s_im[i * LD + col]};
-        cmac(D, gm + conv<R>(d.dterm[i * N + col]), rv);
-        s_re[i * LD + col] = D.re;
-        s_im[i * LD + col] = D.im;
+        if (i < n && col < n) {
+          C D = times_i(C{cre[t][r], cim[t][r]});
+          const C rv{s_re[i * LD + col], s_im[i * LD + col]};
+          cmac(D, gm + conv<R>(d.dterm[i * n + col]), rv);
+          s_re[i * LD + col] = D.re;
+          s_im[i * LD + col] = D.im;
+        }
       }
     }
     __syncthreads();
     // neighbour couplings and the Runge-Kutta epilogue, element-parallel and
     // coalesced, CH elements per thread and pass: every load of a pass is
     // issued before its stores (the stores may alias later loads)
+    // (chunk elements past the block read element 0 and store nothing)
     constexpr int CH = KP <= 4 ? 2 : 1;
-    static_assert(N2 % (kMfmaBS * CH) == 0, "element chunks must tile the ADO");
     const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
-    for (int x0 = threadIdx.x; x0 < N2; x0 += kMfmaBS * CH) {
+    for (int x0 = threadIdx.x; x0 < n2; x0 += kMfmaBS * CH) {
       C Dc[CH];
       PreT<KP> pv[CH];
+      int xs[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int x = x0 + c * kMfmaBS;
-        const int i = x / N, j = x % N;
+        xs[c] = x < n2 ? x : 0;
+        const int i = xs[c] / n, j = xs[c] - (xs[c] / n) * n;
         Dc[c] = C{s_re[i * LD + j], s_im[i * LD + j]};
-        prefetch<F64, KP>(pv[c], A, cur, base + x, true);
+        prefetch<F64, KP>(pv[c], A, cur, base + xs[c], true);
       }
       for (int l = l0; l < l1; ++l) {
         const LinkFlat q = ld_flat(flat + l);
@@ -873,9 +890,8 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
         z128 p0[CH], p1[CH];
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          const int x = x0 + c * kMfmaBS;
-          xv[c] = in[static_cast<long long>(q.nbr) * N2 + x];
-          const long long tb = static_cast<long long>(q.u) * N2 + x;
+          xv[c] = in[static_cast<long long>(q.nbr) * n2 + xs[c]];
+          const long long tb = static_cast<long long>(q.u) * n2 + xs[c];
           if (d.real_v) {
             p0[c] = d.cmac_tab[tb];
           } else {
@@ -901,8 +917,10 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int x = x0 + c * kMfmaBS;
-        add_closure(d, base + x, Dc[c]);
-        err_acc += epilogue<F64, R, KP>(pv[c], A, cur, base + x, Dc[c], pg.alpha);
+        if (x < n2) {
+          add_closure(d, base + x, Dc[c]);
+          err_acc += epilogue<F64, R, KP>(pv[c], A, cur, base + x, Dc[c], pg.alpha);
+        }
       }
     }
     __syncthreads();
@@ -1600,19 +1618,19 @@ template <typename R, bool F64>
 int launch_hilb_mfma(const FormDev& d, const StageArgs& A, hipStream_t st) {
   if (d.mfma == 2) {
     if constexpr (std::is_same<R, float>::value) {
-      return d.N == 32 ? launch_hilb_mfma_n<32, float, false, true>(d, A, st)
+      return d.N <= 32 ? launch_hilb_mfma_n<32, float, false, true>(d, A, st)
                        : launch_hilb_mfma_n<64, float, false, true>(d, A, st);
     } else {
       throw HeomError{HEOM_ERR_UNSUPPORTED, "bf16x3 products need the FP32 state"};
     }
   }
-  return d.N == 32 ? launch_hilb_mfma_n<32, R, F64, false>(d, A, st)
+  return d.N <= 32 ? launch_hilb_mfma_n<32, R, F64, false>(d, A, st)
                    : launch_hilb_mfma_n<64, R, F64, false>(d, A, st);
 }
 
 template <typename R, bool F64>
 int launch_hilb_big(const FormDev& d, const StageArgs& A, hipStream_t st) {
-  if (d.mfma && (d.N == 32 || d.N == 64)) return launch_hilb_mfma<R, F64>(d, A, st);
+  if (d.mfma) return launch_hilb_mfma<R, F64>(d, A, st);  // 16 < N <= 64, diagonal V
   if (!d.diag_v)
     throw HeomError{HEOM_ERR_UNSUPPORTED,
                     "heom_hilb: n_level > 16 is provided for diagonal coupling operators V"};

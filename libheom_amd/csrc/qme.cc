@@ -573,24 +573,31 @@ void Qme::upload() {
   dev.R_ptr = d_R_ptr.as<long long>();
   dev.R_col = d_R_col.as<int>();
   dev.R_val = d_R_val.as<z128>();
-  // matrix-core commutator: heom_hilb, N in {32, 64}, diagonal V
-  const bool mfma_ok = kind == HEOM_HILB && diag_v && (N == 32 || N == 64);
+  // matrix-core commutator: heom_hilb, 16 < N <= 64, diagonal V (tiles of 32 or
+  // 64 rows; H is zero-padded to the tile in its planes)
+  const bool mfma_ok = kind == HEOM_HILB && diag_v && N > 16 && N <= 64;
   dev.mfma = (mfma_ok && opt.matrix_cores) ? (dtype == HEOM_C64_BF16X3 ? 2 : 1) : 0;
   if (dtype == HEOM_C64_BF16X3 && dev.mfma != 2)
     throw HeomError{HEOM_ERR_UNSUPPORTED,
-                    "HEOM_C64_BF16X3 is provided for heom_hilb with n_level 32 or 64, diagonal "
+                    "HEOM_C64_BF16X3 is provided for heom_hilb with 16 < n_level <= 64, diagonal "
                     "coupling operators and matrix_cores = 1"};
   dev.real_h = 1;
   {
-    std::vector<double> hr(N2), hi(N2);
-    std::vector<float> hr32(N2), hi32(N2);
-    for (int e = 0; e < N2; ++e) {
-      hr[e] = H_work_[e].re;
-      hi[e] = H_work_[e].im;
-      hr32[e] = static_cast<float>(hr[e]);
-      hi32[e] = static_cast<float>(hi[e]);
-      if (hi[e] != 0.0) dev.real_h = 0;
-    }
+    // real / imaginary planes of H with leading dimension P (the MFMA tile,
+    // zero-padded), P = N outside the matrix-core path
+    const int P = mfma_ok ? (N <= 32 ? 32 : 64) : N;
+    std::vector<double> hr(static_cast<size_t>(P) * P, 0.0), hi(hr.size(), 0.0);
+    std::vector<float> hr32(hr.size(), 0.f), hi32(hr.size(), 0.f);
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        const z128 h = H_work_[i * N + j];
+        const size_t e = static_cast<size_t>(i) * P + j;
+        hr[e] = h.re;
+        hi[e] = h.im;
+        hr32[e] = static_cast<float>(h.re);
+        hi32[e] = static_cast<float>(h.im);
+        if (h.im != 0.0) dev.real_h = 0;
+      }
     d_Hre.upload(hr);
     d_Him.upload(hi);
     d_Hre32.upload(hr32);

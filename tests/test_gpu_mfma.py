@@ -1,4 +1,5 @@
-"""Matrix-core commutator of heom_hilb (n_level 32 and 64, diagonal V).
+"""Matrix-core commutator of heom_hilb (16 < n_level <= 64, diagonal V; blocks
+other than 32 and 64 are zero-padded to the next tile of 32 or 64 rows).
 
 FP64 (f64 16x16x4 MFMA) must meet the FP64 bar against the oracle (1e-10,
 relative to the magnitude of the result); the FP32 (f32 MFMA) and split-bf16x3
@@ -42,7 +43,7 @@ def rel_err(got, ref):
     return np.abs(got - ref).max() / max(1.0, np.abs(ref).max())
 
 
-@pytest.mark.parametrize("N", [32, 64])
+@pytest.mark.parametrize("N", [17, 24, 32, 33, 48, 63, 64])
 @pytest.mark.parametrize("real_h", [True, False])
 @pytest.mark.parametrize("complex_diag", [False, True])
 @pytest.mark.parametrize("matrix_cores", [True, False])
@@ -73,7 +74,7 @@ def test_calc_diff_dense64_full_config():
     assert rel_err(got, ref) <= TOL
 
 
-@pytest.mark.parametrize("N", [32, 64])
+@pytest.mark.parametrize("N", [20, 32, 48, 64])
 @pytest.mark.parametrize("real_h", [True, False])
 def test_calc_diff_low_precision_variants(N, real_h):
     m = model(N, real_h, False)
@@ -120,6 +121,23 @@ def test_rkdp_dense64_precision_variants_error():
     print("dense64 L=2 rho_S error vs FP64: fp32 %.3e, bf16x3 %.3e" % (e32, e16))
     assert e32 <= 1e-5
     assert e16 <= 1e-3
+
+
+def test_rkdp_padded_block_parity():
+    """N=48 (padded to the 64-row tile), depth 1, Dormand-Prince to t=1: the
+    oracle's accepted / rejected steps and rho_S within 1e-10."""
+    m = configs.dense64(N=48, max_depth=1, t_final=1.0, out_every=0.5)
+    kw = {"dt": m["dt"], "atol": m["atol"], "rtol": m["rtol"]}
+    g = H.heom_hilb(1).load(m, eng())
+    s = H.qme_solver(eng(), g, H.rkdp())
+    rho, rs = s.solve_rho_s(m["rho0"], m["t_list"], kw)
+    o = O.Oracle(O.HILB, 48, 1, n_threads=16).load(m)
+    rho_o, rs_o, st_o, _, _ = o.solve(m["rho0"], m["t_list"], O.RKDP, kw["dt"], kw["atol"],
+                                      kw["rtol"])
+    st = s.stats()
+    assert (st["n_accepted"], st["n_rejected"]) == (st_o.n_accepted, st_o.n_rejected)
+    assert np.abs(rs - rs_o).max() <= TOL
+    assert rel_err(rho, rho_o) <= TOL
 
 
 def test_bf16x3_outside_matrix_core_path_is_unsupported():

@@ -132,20 +132,30 @@ def bench_sweep(H, L, steps, warmup, reps):
         _capi.check(L.heom_solver_stage_bytes(sv._s, p(sby), ctypes.byref(ns)))
         n = ns.value
         gbs = float(sby[:n].mean()) / (float(sms[:n].mean()) * 1e-3) / 1e9
+        final = np.zeros_like(rho0)
+        _capi.check(L.heom_memcpy_d2h(p(final), d, nb))
         L.heom_dev_free(d)
         res[dtype] = {"value": st["ado_stage_updates"] / el, "ms_per_step": 1e3 * el / steps,
                       "stage_ms": float(sms[:n].mean()), "stage_bytes": float(sby[:n].mean()),
                       "hbm_frac": gbs / HBM_PEAK_GBS, "rs": b.split_rho_s(rs),
+                      "final": b.unpack(final),
                       "n_hrchy": b.n_hrchy, "main_size": b.main_size()}
     # members whose FP64 rho_S stays a bounded density matrix (|rho_ij| <= 1);
     # the others grow without bound (RK4 at dt=0.005 is outside its stability
     # region for their largest Pade frequency x depth) and carry no error figure
-    worst, n_div = 0.0, 0
-    for a, r in zip(res["complex64"]["rs"], res["complex128"]["rs"]):
+    # stable members: the whole FP64 hierarchy stays below 1e4 as well (members whose
+    # deep ADOs grow amplify the FP32 rounding even while rho_S stays bounded)
+    worst, n_div, worst_st, n_st = 0.0, 0, 0.0, 0
+    for a, r, full in zip(res["complex64"]["rs"], res["complex128"]["rs"],
+                          res["complex128"]["final"]):
         if not np.isfinite(r).all() or np.abs(r).max() > 1.0 + 1e-6:
             n_div += 1
             continue
-        worst = max(worst, float(np.abs(a - r).max()))
+        e = float(np.abs(a - r).max())
+        worst = max(worst, e)
+        if np.isfinite(full).all() and np.abs(full).max() <= 1e4:
+            n_st += 1
+            worst_st = max(worst_st, e)
     # members one at a time (FP64, same solver): a sample, scaled to 512 by ADO count
     sample = ms[::64]
     t_one, ado_one = 0.0, 0
@@ -169,8 +179,11 @@ def bench_sweep(H, L, steps, warmup, reps):
                  "stage_ms": f64["stage_ms"], "hbm_frac": f64["hbm_frac"]},
         "rho_s_max_abs_err_vs_fp64_bounded_members": worst,
         "members_unbounded_under_rk4": n_div,
+        "stable_members": n_st,
+        "rho_s_max_abs_err_vs_fp64_stable_members": worst_st,
         "fp64_one_at_a_time_est_ms": one_ms,
         "batch_speedup_vs_one_at_a_time_fp64": one_ms / f64["ms_per_step"],
+        "fp32_batch_speedup_vs_one_at_a_time_fp64": one_ms / f32["ms_per_step"],
     }
 
 
@@ -346,6 +359,28 @@ def bench_redfield(H, L, reps, cpu=True):
     return out
 
 
+def profile_stages(H, name, reps):
+    """One configuration's stage kernels only (for rocprofv3 counter passes):
+    the 7-site chain (Dormand-Prince, 12650 ADOs) or the N=64 dense config
+    (Dormand-Prince, 969 ADOs), FP64, every kernel launched directly."""
+    from libheom_amd import _capi, configs
+    L = _capi.load()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    m = configs.exciton_chain7(t_final=20.0) if name == "7site" else configs.dense64(t_final=0.2)
+    eng = H.hip(0)
+    q = H.heom_hilb(m["max_depth"]).load(m, eng)
+    sv = H.qme_solver(eng, q, H.rkdp())
+    sv.set_graphs(False)
+    sv.solve_rho_s(m["rho0"], m["t_list"][:2], {"dt": m["dt"], "atol": m["atol"], "rtol": m["rtol"]})
+    ns = ctypes.c_int()
+    ms, by = np.zeros(8), np.zeros(8)
+    _capi.check(L.heom_solver_time_stages(sv._s, reps, p(ms), ctypes.byref(ns)))
+    _capi.check(L.heom_solver_stage_bytes(sv._s, p(by), ctypes.byref(ns)))
+    n = ns.value
+    print(json.dumps({"config": name, "n_hrchy": q.n_hrchy, "stage_us": [1e3 * x for x in ms[:n]],
+                      "stage_bytes": [float(x) for x in by[:n]]}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,6 +396,10 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--t-final", type=float, default=1000.0,
                     help="propagation length in fs (profiling runs use a shorter one)")
+    ap.add_argument("--profile-config", choices=["7site", "dense64"],
+                    help="counter runs: set up one configuration, run one short solve and "
+                         "--reps timed passes of every stage kernel, print the stage times "
+                         "and exit")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the HBM traffic of the stage kernel from two rocprofv3 "
                          "--pmc counter_collection.csv files and exit")
@@ -381,6 +420,10 @@ def main():
 
     import libheom_amd as H
     from libheom_amd import _capi, configs
+
+    if args.profile_config:
+        profile_stages(H, args.profile_config, args.reps)
+        return
 
     m = configs.exciton_chain7(t_final=args.t_final)
     eng = H.hip(0)

@@ -510,7 +510,9 @@ int heom_solve_dev(heom_solver_handle s, void* rho_dev, const double* t_list, in
         HAMD_HIP(hipMemcpyAsync(dst, cur, sizeof(hamd::z128) * N2, hipMemcpyDeviceToDevice,
                                 sv.q->stream));
     };
-    int rc = sv.solve(rho_dev, t_list, n_t, dt, atol, rtol, out);
+    int rc = (rho_s_out && !batch)
+                 ? sv.solve_gather(rho_dev, t_list, n_t, dt, atol, rtol, rs.p)
+                 : sv.solve(rho_dev, t_list, n_t, dt, atol, rtol, out);
     if (rho_s_out)
       HAMD_HIP(hipMemcpy(rho_s_out, rs.p, sizeof(hamd::z128) * N2 * static_cast<size_t>(n_t),
                          hipMemcpyDeviceToHost));

@@ -237,12 +237,41 @@ struct ResidentStage {
   double ck[2];
   int dst[2];
 };
+// The stage programs of rk4 and lsrk4 as the solver builds them (slots
+// renumbered by first use: rk4 y, sa, acc, sb; lsrk4 y, y_new, buff), compiled
+// into the resident kernel; the solver checks its own program against them.
+struct ResidentProg {
+  int n_stages, n_buf;
+  ResidentStage st[4];
+};
+constexpr ResidentProg kResidentRk4{
+    4, 4,
+    {{0, 1, {0, 0, 0, 0}, 2, {{1.0, 0, 0, 0}, {1.0, 0, 0, 0}}, {0.5, 1.0 / 6.0}, {1, 2}},
+     {1, 2, {0, 2, 0, 0}, 2, {{1.0, 0.0, 0, 0}, {0.0, 1.0, 0, 0}}, {0.5, 2.0 / 6.0}, {3, 2}},
+     {3, 2, {0, 2, 0, 0}, 2, {{1.0, 0.0, 0, 0}, {0.0, 1.0, 0, 0}}, {1.0, 2.0 / 6.0}, {1, 2}},
+     {1, 1, {2, 0, 0, 0}, 1, {{1.0, 0, 0, 0}, {0, 0, 0, 0}}, {1.0 / 6.0, 0.0}, {0, 0}}}};
+constexpr ResidentProg kResidentLsrk4{
+    4, 3,
+    {{0, 1, {0, 0, 0, 0}, 2, {{1.0, 0, 0, 0}, {0.0, 0, 0, 0}}, {1.0 / 3.0, 1.0}, {1, 2}},
+     {1, 2, {1, 2, 0, 0}, 2, {{1.0, 3.0 / 4.0 * -1.0, 0, 0}, {0.0, -1.0, 0, 0}}, {3.0 / 4.0, 1.0}, {0, 2}},
+     {0, 2, {0, 2, 0, 0}, 2, {{1.0, 2.0 / 3.0 * -1.0, 0, 0}, {0.0, -1.0, 0, 0}}, {2.0 / 3.0, 1.0}, {1, 2}},
+     {1, 2, {1, 2, 0, 0}, 2, {{1.0, 1.0 / 4.0 * -1.0, 0, 0}, {0.0, -1.0, 0, 0}}, {1.0 / 4.0, 1.0}, {0, 2}}}};
+
 struct ResidentArgs {
   int n_stages, n_buf;
   ResidentStage st[4];
+  const ResidentStage* prog;  // the same program in device memory (read by the kernel)
   z128* y;          // the FP64 state in HBM (read at the start, written at the end)
   long long n_full; // steps of dt, then one step of dt_last
   double dt, dt_last;
+  // whole solve in one launch (n_int > 0): interval i takes n_full_i steps of dt
+  // and one of dt_last_i; rho_S (the first N^2 elements) is written to
+  // rho_s + i N^2 before interval i and after the last one
+  int n_int;
+  const long long* n_full_i;
+  const double* dt_last_i;
+  z128* rho_s;
+  int scheme;  // 0: the program at prog; 1: kResidentRk4; 2: kResidentLsrk4
 };
 
 // ---------------------------------------------------------------------------

@@ -1414,17 +1414,12 @@ __global__ void __launch_bounds__(1024) rho0_renorm_kernel(z128* y0, z128* y1, z
 // delta term are computed once into registers.  Stage arithmetic follows the
 // epilogue of the stage kernels (k = -dt*D, sum of c_s*operand_s then + ck*k).
 // ---------------------------------------------------------------------------
-template <int MAXL>
+template <int MAXL, int SCHEME>
 __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const ResidentArgs a) {
   extern __shared__ __align__(16) unsigned char res_lds[];
-  __shared__ ResidentStage s_st[4];
   z128* buf = reinterpret_cast<z128*>(res_lds);
-  if (threadIdx.x == 0) {  // constant indices: the argument block stays in the constant space
-    s_st[0] = a.st[0];
-    s_st[1] = a.st[1];
-    s_st[2] = a.st[2];
-    s_st[3] = a.st[3];
-  }
+  // the stage program: uniform reads through a read-only pointer (scalar loads)
+  const ResidentStage* __restrict__ prog = a.prog;
   const int ms = static_cast<int>(d.main_size);
   const int N = d.N, N2 = d.N2;
   z128* s_H = buf + static_cast<size_t>(a.n_buf) * ms;
@@ -1459,47 +1454,60 @@ __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const R
     }
   }
   __syncthreads();
-  const long long n_steps = a.n_full + 1;
-  for (long long step = 0; step < n_steps; ++step) {
-    const double dt = step < a.n_full ? a.dt : a.dt_last;
-    for (int s = 0; s < a.n_stages; ++s) {
-      const ResidentStage& st = s_st[s];
-      if (ok) {
-        const z128* X = buf + static_cast<size_t>(st.in) * ms;
-        const z128* Xg = X + static_cast<size_t>(g) * N2;
-        z128 acc{0.0, 0.0};
-        for (int k = 0; k < N; ++k) {
-          cmac(acc, s_H[i * N + k], Xg[k * N + j]);
-          const z128 t = Xg[i * N + k];
-          cmac(acc, z128{-t.re, -t.im}, s_H[k * N + j]);
-        }
-        z128 D = times_i(acc);
-        cmac(D, diag, X[e]);
-#pragma unroll
-        for (int l = 0; l < MAXL; ++l)
-          if (l < nl) cmac(D, cf[l], X[nbr[l]]);
-        const z128 k{-(dt * D.re - 0.0 * D.im), -(dt * D.im + 0.0 * D.re)};
-        z128 pv[kResidentPre];
-#pragma unroll
-        for (int p = 0; p < kResidentPre; ++p)
-          pv[p] = p < st.npre ? buf[static_cast<size_t>(st.pre[p]) * ms + e] : z128{0.0, 0.0};
-        for (int c = 0; c < st.ncombo; ++c) {
-          z128 v{0.0, 0.0};
-#pragma unroll
-          for (int p = 0; p < kResidentPre; ++p) {
-            if (p < st.npre) {
-              v.re = fma(st.cc[c][p], pv[p].re, v.re);
-              v.im = fma(st.cc[c][p], pv[p].im, v.im);
-            }
+  const int n_int = a.n_int > 0 ? a.n_int : 1;
+  for (int iv = 0; iv < n_int; ++iv) {
+    const long long n_full = a.n_int > 0 ? a.n_full_i[iv] : a.n_full;
+    const double dt_last = a.n_int > 0 ? a.dt_last_i[iv] : a.dt_last;
+    if (a.n_int > 0 && e < N2) a.rho_s[static_cast<size_t>(iv) * N2 + e] = buf[e];
+    const long long n_steps = n_full + 1;
+    for (long long step = 0; step < n_steps; ++step) {
+      const double dt = step < n_full ? a.dt : dt_last;
+      auto stage = [&](const ResidentStage& st) {
+        if (ok) {
+          const z128* X = buf + static_cast<size_t>(st.in) * ms;
+          const z128* Xg = X + static_cast<size_t>(g) * N2;
+          z128 acc{0.0, 0.0};
+          for (int k = 0; k < N; ++k) {
+            cmac(acc, s_H[i * N + k], Xg[k * N + j]);
+            const z128 t = Xg[i * N + k];
+            cmac(acc, z128{-t.re, -t.im}, s_H[k * N + j]);
           }
-          v.re = fma(st.ck[c], k.re, v.re);
-          v.im = fma(st.ck[c], k.im, v.im);
-          buf[static_cast<size_t>(st.dst[c]) * ms + e] = v;
+          z128 D = times_i(acc);
+          cmac(D, diag, X[e]);
+#pragma unroll
+          for (int l = 0; l < MAXL; ++l)
+            if (l < nl) cmac(D, cf[l], X[nbr[l]]);
+          const z128 k{-(dt * D.re - 0.0 * D.im), -(dt * D.im + 0.0 * D.re)};
+          z128 pv[kResidentPre];
+#pragma unroll
+          for (int p = 0; p < kResidentPre; ++p)
+            pv[p] = p < st.npre ? buf[static_cast<size_t>(st.pre[p]) * ms + e] : z128{0.0, 0.0};
+          for (int c = 0; c < st.ncombo; ++c) {
+            z128 v{0.0, 0.0};
+#pragma unroll
+            for (int p = 0; p < kResidentPre; ++p) {
+              if (p < st.npre) {
+                v.re = fma(st.cc[c][p], pv[p].re, v.re);
+                v.im = fma(st.cc[c][p], pv[p].im, v.im);
+              }
+            }
+            v.re = fma(st.ck[c], k.re, v.re);
+            v.im = fma(st.ck[c], k.im, v.im);
+            buf[static_cast<size_t>(st.dst[c]) * ms + e] = v;
+          }
         }
+        __syncthreads();
+      };
+      if constexpr (SCHEME == 0) {
+        for (int s = 0; s < a.n_stages; ++s) stage(prog[s]);
+      } else {
+        constexpr const ResidentProg& P = SCHEME == 1 ? kResidentRk4 : kResidentLsrk4;
+#pragma unroll
+        for (int s = 0; s < P.n_stages; ++s) stage(P.st[s]);
       }
-      __syncthreads();
     }
   }
+  if (a.n_int > 0 && e < N2) a.rho_s[static_cast<size_t>(a.n_int) * N2 + e] = buf[e];
   if (ok) a.y[e] = buf[e];
 }
 
@@ -1801,21 +1809,31 @@ bool resident_ok(const FormDev& d, int max_links) {
 
 void launch_resident(const FormDev& d, int max_links, const ResidentArgs& a, hipStream_t stream) {
   const int ms = static_cast<int>(d.main_size);
-  if (!resident_ok(d, max_links) || a.n_stages < 1 || a.n_stages > 4 || a.n_buf < 1)
+  if (!resident_ok(d, max_links) || a.n_stages < 1 || a.n_stages > 4 || a.n_buf < 1 || !a.prog)
     throw HeomError{HEOM_ERR_STATE, "resident integrator: unsupported problem"};
   const size_t lds = sizeof(z128) * (static_cast<size_t>(a.n_buf) * ms + d.N2);
   if (lds > kResidentLds) throw HeomError{HEOM_ERR_STATE, "resident integrator: LDS exceeded"};
   const int bs = ((ms + 63) / 64) * 64;
-#define HAMD_RES(L)                                                                   \
+#define HAMD_RES(L, S)                                                                \
   do {                                                                                \
-    static bool once = (allow_lds(resident_kernel<L>, kResidentLds), true);           \
+    static bool once = (allow_lds(resident_kernel<L, S>, kResidentLds), true);        \
     (void)once;                                                                       \
-    hipLaunchKernelGGL(resident_kernel<L>, dim3(1), dim3(bs), lds, stream, d, a);      \
+    hipLaunchKernelGGL((resident_kernel<L, S>), dim3(1), dim3(bs), lds, stream, d, a); \
+  } while (0)
+#define HAMD_RES_S(L)       \
+  do {                      \
+    if (a.scheme == 1)      \
+      HAMD_RES(L, 1);       \
+    else if (a.scheme == 2) \
+      HAMD_RES(L, 2);       \
+    else                    \
+      HAMD_RES(L, 0);       \
   } while (0)
   if (max_links <= 4)
-    HAMD_RES(4);
+    HAMD_RES_S(4);
   else
-    HAMD_RES(8);
+    HAMD_RES_S(8);
+#undef HAMD_RES_S
 #undef HAMD_RES
   HAMD_HIP(hipGetLastError());
 }

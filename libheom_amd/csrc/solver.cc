@@ -293,6 +293,26 @@ void Solver::build_resident() {
   r.n_buf = static_cast<int>(bmap.size());
   const size_t lds = sizeof(z128) * (static_cast<size_t>(r.n_buf) * ms + q->N2);
   if (lds > kResidentLds) return;
+  std::vector<ResidentStage> prog(r.st, r.st + r.n_stages);
+  d_res_prog.upload(prog);
+  r.prog = d_res_prog.as<ResidentStage>();
+  // the compiled-in program when the solver's equals it field by field
+  auto same = [&](const ResidentProg& P) {
+    if (P.n_stages != r.n_stages || P.n_buf != r.n_buf) return false;
+    for (int s = 0; s < r.n_stages; ++s) {
+      const ResidentStage &x = P.st[s], &y = r.st[s];
+      if (x.in != y.in || x.npre != y.npre || x.ncombo != y.ncombo) return false;
+      for (int p = 0; p < x.npre; ++p)
+        if (x.pre[p] != y.pre[p]) return false;
+      for (int c = 0; c < x.ncombo; ++c) {
+        if (x.ck[c] != y.ck[c] || x.dst[c] != y.dst[c]) return false;
+        for (int p = 0; p < x.npre; ++p)
+          if (x.cc[c][p] != y.cc[c][p]) return false;
+      }
+    }
+    return true;
+  };
+  r.scheme = same(kResidentRk4) ? 1 : same(kResidentLsrk4) ? 2 : 0;
   res = r;
 }
 
@@ -367,9 +387,9 @@ void Solver::replay(std::map<long long, std::unique_ptr<GraphExec>>& cache, long
 // dt, the last one shortened to end exactly at t1.  The step sequence is
 // computed on the host with the reference's arithmetic; the steps of the full
 // dt are replayed from graphs of kGraphSteps steps and of one step.
-void Solver::rk4_interval(double t0, double t1, double dt) {
+void Solver::fixed_steps(double t0, double t1, double dt, long long& n_full, double& dt_last) {
   double t = t0, dt_1 = dt;
-  long long n_full = 0;
+  n_full = 0;
   while (true) {
     if (t + dt_1 - t1 > -std::numeric_limits<double>::epsilon()) {
       dt_1 = t1 - t;
@@ -378,6 +398,13 @@ void Solver::rk4_interval(double t0, double t1, double dt) {
     ++n_full;
     t += dt_1;
   }
+  dt_last = dt_1;
+}
+
+void Solver::rk4_interval(double t0, double t1, double dt) {
+  long long n_full = 0;
+  double dt_1 = dt;
+  fixed_steps(t0, t1, dt, n_full, dt_1);
   if (res.n_stages > 0) {  // the whole interval in one launch
     ResidentArgs a = res;
     a.y = static_cast<z128*>(P[SL_Y].p);
@@ -497,6 +524,59 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
   stats.ado_stage_updates = stats.n_rhs * q->hs.n_hrchy;
   stats.gave_up |= gave_up;
   return gave_up ? HEOM_ERR_STEP_REJECTED : HEOM_OK;
+}
+
+int Solver::solve_gather(void* rho_dev, const double* t_list, int n_t, double dt, double atol,
+                         double rtol, void* rho_s_dev) {
+  const size_t rs_bytes = sizeof(z128) * q->N2;
+  hipStream_t st = q->stream;
+  if (kind == HEOM_RKDP || n_t < 2 || q->kind == kBatch) {
+    return solve(rho_dev, t_list, n_t, dt, atol, rtol, [&](int i, const void* cur) {
+      HAMD_HIP(hipMemcpyAsync(static_cast<char*>(rho_s_dev) + rs_bytes * i, cur, rs_bytes,
+                              hipMemcpyDeviceToDevice, st));
+    });
+  }
+  for (int i = 0; i + 1 < n_t; ++i)
+    if (!(t_list[i + 1] >= t_list[i]))
+      throw HeomError{HEOM_ERR_ARG, "solve: t_list must be non-decreasing"};
+  if (!(dt > 0.0)) throw HeomError{HEOM_ERR_ARG, "rk4/lsrk4: dt must be > 0"};
+  HAMD_HIP(hipSetDevice(q->opt.device));
+  build_stages(atol, rtol);
+  build_resident();
+  if (res.n_stages == 0) {
+    reset_graphs();
+    return solve(rho_dev, t_list, n_t, dt, atol, rtol, [&](int i, const void* cur) {
+      HAMD_HIP(hipMemcpyAsync(static_cast<char*>(rho_s_dev) + rs_bytes * i, cur, rs_bytes,
+                              hipMemcpyDeviceToDevice, st));
+    });
+  }
+  // the whole solve in one resident launch
+  std::vector<long long> nf(n_t - 1);
+  std::vector<double> dl(n_t - 1);
+  long long steps = 0;
+  for (int i = 0; i + 1 < n_t; ++i) {
+    fixed_steps(t_list[i], t_list[i + 1], dt, nf[i], dl[i]);
+    steps += nf[i] + 1;
+  }
+  d_res_nf.upload(nf);
+  d_res_dl.upload(dl);
+  ResidentArgs a = res;
+  a.y = static_cast<z128*>(P[SL_Y].p);
+  a.dt = dt;
+  a.n_int = n_t - 1;
+  a.n_full_i = d_res_nf.as<long long>();
+  a.dt_last_i = d_res_dl.as<double>();
+  a.rho_s = static_cast<z128*>(rho_s_dev);
+  const size_t bytes = sizeof(z128) * ms;
+  HAMD_HIP(hipMemcpyAsync(P[SL_Y].p, rho_dev, bytes, hipMemcpyDeviceToDevice, st));
+  launch_resident(q->dev, q->dev.max_links, a, st);
+  HAMD_HIP(hipMemcpyAsync(rho_dev, P[SL_Y].p, bytes, hipMemcpyDeviceToDevice, st));
+  HAMD_HIP(hipStreamSynchronize(st));
+  stats.n_rhs += 4 * steps;
+  stats.n_attempts += steps;
+  stats.n_accepted += steps;
+  stats.ado_stage_updates = stats.n_rhs * q->hs.n_hrchy;
+  return HEOM_OK;
 }
 
 void Solver::time_stages(int reps, std::vector<double>& out) {

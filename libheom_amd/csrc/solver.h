@@ -62,6 +62,11 @@ class Solver {
   // is called at every t_i with a device pointer to the FP64 state.
   int solve(void* rho_dev, const double* t_list, int n_t, double dt, double atol, double rtol,
             const std::function<void(int, const void*)>& on_output);
+  // The same with rho_S of every output time gathered into rho_s_dev (n_t N^2
+  // complex128 in device memory) instead of a callback; a small fixed-step
+  // problem then runs the whole solve in one resident launch.
+  int solve_gather(void* rho_dev, const double* t_list, int n_t, double dt, double atol,
+                   double rtol, void* rho_s_dev);
 
   // Average duration (ms) of each stage kernel over `reps` replays.
   void time_stages(int reps, std::vector<double>& ms_out);
@@ -103,6 +108,10 @@ class Solver {
   // resident integrator (small heom_hilb problems): stage program built by
   // build_stages when the problem qualifies, else n_stages == 0
   ResidentArgs res{};
+  DevBuf d_res_prog, d_res_nf, d_res_dl;
+  // fixed_step_size_solver::solve_1's step sequence over [t0, t1]: n_full steps
+  // of dt, then one step of dt_last
+  static void fixed_steps(double t0, double t1, double dt, long long& n_full, double& dt_last);
   void build_resident();
   int rkdp_interval(double t0, double t1, double dt_first, int& batch);
 };

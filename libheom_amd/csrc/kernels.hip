@@ -788,7 +788,7 @@ __device__ __forceinline__ void mfma_commutator_bf16x3(const FormDev& d, const f
 // (32 or 64); the ADO block is n x n with n = d.N in (N/2, N] for N = 64, in
 // (16, 32] for N = 32, zero-padded in LDS (and H in its planes), so that the
 // padded rows and columns of the products are exactly zero.
-template <int N, typename R, bool REALH, bool F64, int KP, bool BF16X3>
+template <int N, typename R, bool REALH, bool F64, int KP, bool BF16X3, bool PADDED>
 __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   using M = Mfma<R>;
@@ -799,8 +799,9 @@ __global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, con
   R* s_im = s_re + N * LD;
   const Prologue pg = prologue(A);
   if (pg.skip) return;
-  const int n = d.N, n2 = d.N2;  // the ADO block (n <= N)
-  if (n < N) {  // the padding stays zero: only rows and columns < n are ever written
+  // the ADO block (n <= N); unpadded blocks keep the index arithmetic constant
+  const int n = PADDED ? d.N : N, n2 = PADDED ? d.N2 : N * N;
+  if (PADDED) {  // the padding stays zero: only rows and columns < n are ever written
     for (int x = threadIdx.x; x < 2 * N * LD; x += kMfmaBS) s_re[x] = R(0);
     __syncthreads();
   }
@@ -1606,18 +1607,23 @@ int launch_hilb_mfma_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
   const size_t lds = 2 * sizeof(R) * N * MfmaCfg<N>::LD;
   const int grid = cap_grid(d.n_hrchy);
   const bool small = A.npre <= 4 && !A.do_err;
-#define HAMD_MFMA(REALH, KP)                                                     \
+#define HAMD_MFMA(REALH, KP, PAD)                                                \
   {                                                                              \
-    auto k = hilb_mfma_kernel<N, R, REALH, F64, KP, BF16X3>;                     \
+    auto k = hilb_mfma_kernel<N, R, REALH, F64, KP, BF16X3, PAD>;                \
     static bool once = (allow_lds(k, lds), true);                                \
     (void)once;                                                                  \
     hipLaunchKernelGGL(k, dim3(grid), dim3(kMfmaBS), lds, st, d, A);             \
   }
-  if (d.real_h) {
-    if (small) HAMD_MFMA(true, 4) else HAMD_MFMA(true, kMaxPre)
-  } else {
-    if (small) HAMD_MFMA(false, 4) else HAMD_MFMA(false, kMaxPre)
+#define HAMD_MFMA_P(REALH, KP)                          \
+  {                                                     \
+    if (d.N == N) HAMD_MFMA(REALH, KP, false) else HAMD_MFMA(REALH, KP, true) \
   }
+  if (d.real_h) {
+    if (small) HAMD_MFMA_P(true, 4) else HAMD_MFMA_P(true, kMaxPre)
+  } else {
+    if (small) HAMD_MFMA_P(false, 4) else HAMD_MFMA_P(false, kMaxPre)
+  }
+#undef HAMD_MFMA_P
 #undef HAMD_MFMA
   return grid;
 }

@@ -784,12 +784,15 @@ __device__ __forceinline__ void mfma_commutator_bf16x3(const FormDev& d, const f
   }
 }
 
+// FP32 operands (complex64, BF16X3) ask for two waves per SIMD (measured 17% /
+// 5% faster at N=64); FP64 keeps one (the bound costs it 3%).
 // BF16X3: R = float, products via mfma_commutator_bf16x3.  N is the tile size
 // (32 or 64); the ADO block is n x n with n = d.N in (N/2, N] for N = 64, in
 // (16, 32] for N = 32, zero-padded in LDS (and H in its planes), so that the
 // padded rows and columns of the products are exactly zero.
 template <int N, typename R, bool REALH, bool F64, int KP, bool BF16X3, bool PADDED>
-__global__ void __launch_bounds__(kMfmaBS) hilb_mfma_kernel(const FormDev d, const StageArgs A) {
+__global__ void __launch_bounds__(kMfmaBS, sizeof(R) == 4 ? 2 : 1)
+    hilb_mfma_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   using M = Mfma<R>;
   using Cfg = MfmaCfg<N>;

@@ -221,6 +221,20 @@ struct FormDev {
   const BatchItem* items;
 };
 
+// Dormand-Prince 5(4) tableau (rkdp.h:21-57): A (stage inputs), B (5th-order
+// solution), E (B minus the embedded 4th-order weights, for the error)
+constexpr double kDpA[6][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {1.0 / 5, 0, 0, 0, 0, 0},
+    {3.0 / 40, 9.0 / 40, 0, 0, 0, 0},
+    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0},
+    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0},
+    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0}};
+constexpr double kDpB[6] = {35.0 / 384, 0.0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784,
+                            11.0 / 84};
+constexpr double kDpE[7] = {-71.0 / 57600, 0.0, 71.0 / 16695, -71.0 / 1920,
+                            17253.0 / 339200, -22.0 / 525, 1.0 / 40};
+
 // Resident fixed-step integrator (hierarchies of at most kResidentMax
 // elements): the stage program of a fixed-step scheme with the solver's slots
 // renumbered to LDS buffers 0..n_buf-1 (buffer 0 holds the state), run for a
@@ -272,6 +286,21 @@ struct ResidentArgs {
   const double* dt_last_i;
   z128* rho_s;
   int scheme;  // 0: the program at prog; 1: kResidentRk4; 2: kResidentLsrk4
+};
+
+// Resident Dormand-Prince: the whole solve of a small problem in one launch
+// (same eligibility as the fixed-step integrator, at most kResidentDopriMax
+// elements: LDS holds y, y_new, k1..k6 and two stage inputs).
+constexpr int kResidentDopriBuf = 10;
+constexpr int kResidentDopriMax = 960;
+struct ResidentDopriArgs {
+  z128* y;               // FP64 state in HBM (read at the start, written at the end)
+  int n_int;             // output intervals
+  const double* t_list;  // n_int + 1 output times, device memory
+  double dt_first;       // dt of the first attempt (dt_save, else the caller's dt)
+  double atol, rtol, inv_n;
+  z128* rho_s;           // rho_S at the n_int + 1 output times
+  Ctrl* ctrl;            // the final control block (counters, dt_save, any give-up)
 };
 
 // ---------------------------------------------------------------------------

@@ -44,6 +44,11 @@ void launch_gather_rho_s(const FormDev& d, void* dst, const void* src, hipStream
 // a whole output interval.  resident_ok tells whether a problem qualifies.
 bool resident_ok(const FormDev& d, int max_links);
 void launch_resident(const FormDev& d, int max_links, const ResidentArgs& a, hipStream_t stream);
+// Resident Dormand-Prince (the fixed-step conditions and at most
+// kResidentDopriMax elements): the whole adaptive solve in one launch.
+bool resident_dopri_ok(const FormDev& d, int max_links);
+void launch_resident_dopri(const FormDev& d, int max_links, const ResidentDopriArgs& a,
+                           hipStream_t stream);
 // Tr rho_0 (real part) of the FP64 state y into *tr0
 void launch_rho0_trace(const void* y, int N, double* tr0, hipStream_t stream);
 // rho_0 <- (rho_0 + rho_0^+)/2 * tr0 / Tr rho_0 in the FP64 state of parity

@@ -1418,6 +1418,61 @@ __global__ void __launch_bounds__(1024) rho0_renorm_kernel(z128* y0, z128* y1, z
 // delta term are computed once into registers.  Stage arithmetic follows the
 // epilogue of the stage kernels (k = -dt*D, sum of c_s*operand_s then + ck*k).
 // ---------------------------------------------------------------------------
+// One lane's element of a resident problem: the link coefficients and
+// neighbour elements (computed once) and the operator D at that element.
+template <int MAXL>
+struct ResidentElem {
+  int g, i, j, nl;
+  z128 diag;
+  int nbr[MAXL];
+  z128 cf[MAXL];
+
+  __device__ __forceinline__ void init(const FormDev& d, int e, bool ok) {
+    const int N = d.N, N2 = d.N2;
+    g = ok ? e / N2 : 0;
+    const int ee = ok ? e - g * N2 : 0;
+    i = ee / N;
+    j = ee - (ee / N) * N;
+    const int l0 = d.link_ptr[g], l1 = d.link_ptr[g + 1];
+    nl = ok ? l1 - l0 : 0;
+    diag = d.gam[g] + d.dterm[ee];
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l) {
+      nbr[l] = 0;
+      cf[l] = z128{0.0, 0.0};
+      if (l < nl) {
+        const LinkFlat q = d.flat[l0 + l];
+        nbr[l] = q.nbr * N2 + ee;
+        const long long b = static_cast<long long>(q.u) * N2 + ee;
+        if (d.real_v) {
+          const z128 p = d.cmac_tab[b];
+          cf[l].re = p.re * q.iphi.re + p.im * q.psi.re + q.dir.re;
+          cf[l].im = p.re * q.iphi.im + p.im * q.psi.im + q.dir.im;
+        } else {
+          cf[l] = d.cmac_tab[2 * b] * q.iphi + d.cmac_tab[2 * b + 1] * q.psi + q.dir;
+        }
+      }
+    }
+  }
+
+  // D at element e of the state X (same expression order as hilb_fast_kernel)
+  __device__ __forceinline__ z128 op(const z128* X, const z128* s_H, int e, int N, int N2) const {
+    const z128* Xg = X + static_cast<size_t>(g) * N2;
+    z128 acc{0.0, 0.0};
+    for (int k = 0; k < N; ++k) {
+      cmac(acc, s_H[i * N + k], Xg[k * N + j]);
+      const z128 t = Xg[i * N + k];
+      cmac(acc, z128{-t.re, -t.im}, s_H[k * N + j]);
+    }
+    z128 D = times_i(acc);
+    cmac(D, diag, X[e]);
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l)
+      if (l < nl) cmac(D, cf[l], X[nbr[l]]);
+    return D;
+  }
+};
+
 template <int MAXL, int SCHEME>
 __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const ResidentArgs a) {
   extern __shared__ __align__(16) unsigned char res_lds[];
@@ -1432,31 +1487,8 @@ __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const R
   for (int x = e; x < a.n_buf * ms; x += blockDim.x)
     buf[x] = x < ms ? a.y[x] : z128{0.0, 0.0};
   for (int x = e; x < N2; x += blockDim.x) s_H[x] = d.H[x];
-  const int g = ok ? e / N2 : 0;
-  const int ee = ok ? e - g * N2 : 0;
-  const int i = ee / N, j = ee - (ee / N) * N;
-  const int l0 = d.link_ptr[g], l1 = d.link_ptr[g + 1];
-  const int nl = ok ? l1 - l0 : 0;
-  const z128 diag = d.gam[g] + d.dterm[ee];
-  int nbr[MAXL];
-  z128 cf[MAXL];
-#pragma unroll
-  for (int l = 0; l < MAXL; ++l) {
-    nbr[l] = 0;
-    cf[l] = z128{0.0, 0.0};
-    if (l < nl) {
-      const LinkFlat q = d.flat[l0 + l];
-      nbr[l] = q.nbr * N2 + ee;
-      const long long b = static_cast<long long>(q.u) * N2 + ee;
-      if (d.real_v) {
-        const z128 p = d.cmac_tab[b];
-        cf[l].re = p.re * q.iphi.re + p.im * q.psi.re + q.dir.re;
-        cf[l].im = p.re * q.iphi.im + p.im * q.psi.im + q.dir.im;
-      } else {
-        cf[l] = d.cmac_tab[2 * b] * q.iphi + d.cmac_tab[2 * b + 1] * q.psi + q.dir;
-      }
-    }
-  }
+  ResidentElem<MAXL> el;
+  el.init(d, e, ok);
   __syncthreads();
   const int n_int = a.n_int > 0 ? a.n_int : 1;
   for (int iv = 0; iv < n_int; ++iv) {
@@ -1468,19 +1500,7 @@ __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const R
       const double dt = step < n_full ? a.dt : dt_last;
       auto stage = [&](const ResidentStage& st) {
         if (ok) {
-          const z128* X = buf + static_cast<size_t>(st.in) * ms;
-          const z128* Xg = X + static_cast<size_t>(g) * N2;
-          z128 acc{0.0, 0.0};
-          for (int k = 0; k < N; ++k) {
-            cmac(acc, s_H[i * N + k], Xg[k * N + j]);
-            const z128 t = Xg[i * N + k];
-            cmac(acc, z128{-t.re, -t.im}, s_H[k * N + j]);
-          }
-          z128 D = times_i(acc);
-          cmac(D, diag, X[e]);
-#pragma unroll
-          for (int l = 0; l < MAXL; ++l)
-            if (l < nl) cmac(D, cf[l], X[nbr[l]]);
+          const z128 D = el.op(buf + static_cast<size_t>(st.in) * ms, s_H, e, N, N2);
           const z128 k{-(dt * D.re - 0.0 * D.im), -(dt * D.im + 0.0 * D.re)};
           z128 pv[kResidentPre];
 #pragma unroll
@@ -1513,6 +1533,138 @@ __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const R
   }
   if (a.n_int > 0 && e < N2) a.rho_s[static_cast<size_t>(a.n_int) * N2 + e] = buf[e];
   if (ok) a.y[e] = buf[e];
+}
+
+// Resident Dormand-Prince: the whole adaptive solve of a small heom_hilb
+// problem (diagonal V, FP64) in one workgroup.  Stage arithmetic follows the
+// stage kernels (k = -dt*D; y + sum a_ij k_j as one fma chain in operand order;
+// the error combination and weights of rkdp.h:136-153), the step control is
+// dopri_control on a control block in LDS, and each output interval starts as
+// ctrl_init_kernel does.  The error norm is summed in a fixed order (wavefront
+// shuffles, then the waves in order).  LDS buffers: 0/1 the y / y_new pair
+// (swapped by Ctrl::cur on acceptance), 2..7 k1..k6, 8/9 the stage inputs.
+template <int MAXL>
+__global__ void __launch_bounds__(1024)
+    resident_dopri_kernel(const FormDev d, const ResidentDopriArgs a) {
+  extern __shared__ __align__(16) unsigned char res_lds[];
+  __shared__ Ctrl s_c;
+  __shared__ double s_red[16];
+  __shared__ int s_gave_up;
+  z128* buf = reinterpret_cast<z128*>(res_lds);
+  const int ms = static_cast<int>(d.main_size);
+  const int N = d.N, N2 = d.N2;
+  z128* s_H = buf + static_cast<size_t>(kResidentDopriBuf) * ms;
+  const int e = threadIdx.x;
+  const bool ok = e < ms;
+  for (int x = e; x < kResidentDopriBuf * ms; x += blockDim.x)
+    buf[x] = x < ms ? a.y[x] : z128{0.0, 0.0};
+  for (int x = e; x < N2; x += blockDim.x) s_H[x] = d.H[x];
+  ResidentElem<MAXL> el;
+  el.init(d, e, ok);
+  if (e == 0) {
+    s_c = Ctrl{};
+    s_c.dt = a.dt_first;
+    s_gave_up = 0;
+  }
+  __syncthreads();
+  if (e < N2) a.rho_s[e] = buf[e];
+  z128* K = buf + 2 * static_cast<size_t>(ms);
+  z128* SA = buf + 8 * static_cast<size_t>(ms);
+  z128* SB = buf + 9 * static_cast<size_t>(ms);
+  for (int iv = 0; iv < a.n_int; ++iv) {
+    if (e == 0) {  // ctrl_init_kernel: dt carries over from the last attempt
+      const double t0 = a.t_list[iv], t1 = a.t_list[iv + 1];
+      double dt = s_c.dt;
+      s_c.t = t0;
+      s_c.t_bound = t1;
+      s_c.done = 0;
+      s_c.count = 1;
+      s_c.gave_up = 0;
+      if (t0 + dt > t1) dt = t1 - t0;
+      s_c.dt = dt;
+    }
+    __syncthreads();
+    while (!s_c.done) {
+      const double dt = s_c.dt;
+      const int cur = s_c.cur;
+      const z128* Y = buf + (cur ? ms : 0);
+      z128* YN = buf + (cur ? 0 : ms);
+#pragma unroll
+      for (int st = 0; st < 6; ++st) {
+        const z128* in = st == 0 ? Y : ((st & 1) ? SA : SB);
+        z128* out = st == 5 ? YN : ((st & 1) ? SB : SA);
+        if (ok) {
+          const z128 D = el.op(in, s_H, e, N, N2);
+          const z128 k{-(dt * D.re - 0.0 * D.im), -(dt * D.im + 0.0 * D.re)};
+          K[static_cast<size_t>(st) * ms + e] = k;
+          const z128 y = Y[e];
+          z128 v{fma(1.0, y.re, 0.0), fma(1.0, y.im, 0.0)};
+          if (st < 5) {
+#pragma unroll
+            for (int j = 0; j < st; ++j) {
+              const z128 kj = K[static_cast<size_t>(j) * ms + e];
+              v.re = fma(kDpA[st + 1][j], kj.re, v.re);
+              v.im = fma(kDpA[st + 1][j], kj.im, v.im);
+            }
+            v.re = fma(kDpA[st + 1][st], k.re, v.re);
+            v.im = fma(kDpA[st + 1][st], k.im, v.im);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+              if (kDpB[j] != 0.0) {
+                const z128 kj = K[static_cast<size_t>(j) * ms + e];
+                v.re = fma(kDpB[j], kj.re, v.re);
+                v.im = fma(kDpB[j], kj.im, v.im);
+              }
+            }
+            v.re = fma(kDpB[5], k.re, v.re);
+            v.im = fma(kDpB[5], k.im, v.im);
+          }
+          out[e] = v;
+        }
+        __syncthreads();
+      }
+      double w2 = 0.0;
+      if (ok) {
+        const z128 D = el.op(YN, s_H, e, N, N2);
+        const z128 k{-(dt * D.re - 0.0 * D.im), -(dt * D.im + 0.0 * D.re)};
+        z128 er{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          if (kDpE[j] != 0.0) {
+            const z128 kj = K[static_cast<size_t>(j) * ms + e];
+            er.re = fma(kDpE[j], kj.re, er.re);
+            er.im = fma(kDpE[j], kj.im, er.im);
+          }
+        }
+        // the stage kernels' padded sixth operand adds 0*0
+        er.re = fma(0.0, 0.0, er.re);
+        er.im = fma(0.0, 0.0, er.im);
+        er.re = fma(kDpE[6], k.re, er.re);
+        er.im = fma(kDpE[6], k.im, er.im);
+        const double a1 = zabs(YN[e]), a0 = zabs(Y[e]);
+        const double w = zabs(er) / (a.atol + a.rtol * (a1 > a0 ? a1 : a0));
+        w2 = w * w;
+      }
+      for (int off = 32; off > 0; off >>= 1) w2 += __shfl_down(w2, off, 64);
+      if ((e & 63) == 0) s_red[e >> 6] = w2;
+      __syncthreads();
+      if (e == 0) {
+        double t = 0.0;
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) t += s_red[w];
+        dopri_control(&s_c, sqrt(t * a.inv_n));
+        if (s_c.gave_up) s_gave_up = 1;
+      }
+      __syncthreads();
+    }
+    if (e < N2) a.rho_s[static_cast<size_t>(iv + 1) * N2 + e] = buf[(s_c.cur ? ms : 0) + e];
+  }
+  if (ok) a.y[e] = buf[(s_c.cur ? ms : 0) + e];
+  if (e == 0) {
+    Ctrl c = s_c;
+    c.gave_up = s_gave_up;
+    *a.ctrl = c;
+  }
 }
 
 __global__ void ctrl_init_kernel(Ctrl* c, double t0, double t_bound, double dt) {
@@ -1844,6 +1996,32 @@ void launch_resident(const FormDev& d, int max_links, const ResidentArgs& a, hip
     HAMD_RES_S(8);
 #undef HAMD_RES_S
 #undef HAMD_RES
+  HAMD_HIP(hipGetLastError());
+}
+
+bool resident_dopri_ok(const FormDev& d, int max_links) {
+  return resident_ok(d, max_links) && d.main_size <= kResidentDopriMax;
+}
+
+void launch_resident_dopri(const FormDev& d, int max_links, const ResidentDopriArgs& a,
+                           hipStream_t stream) {
+  if (!resident_dopri_ok(d, max_links) || a.n_int < 1)
+    throw HeomError{HEOM_ERR_STATE, "resident Dormand-Prince: unsupported problem"};
+  const int ms = static_cast<int>(d.main_size);
+  const size_t lds = sizeof(z128) * (static_cast<size_t>(kResidentDopriBuf) * ms + d.N2);
+  if (lds > kResidentLds) throw HeomError{HEOM_ERR_STATE, "resident Dormand-Prince: LDS exceeded"};
+  const int bs = ((ms + 63) / 64) * 64;
+#define HAMD_RDP(L)                                                                      \
+  do {                                                                                   \
+    static bool once = (allow_lds(resident_dopri_kernel<L>, kResidentLds), true);        \
+    (void)once;                                                                          \
+    hipLaunchKernelGGL((resident_dopri_kernel<L>), dim3(1), dim3(bs), lds, stream, d, a); \
+  } while (0)
+  if (max_links <= 4)
+    HAMD_RDP(4);
+  else
+    HAMD_RDP(8);
+#undef HAMD_RDP
   HAMD_HIP(hipGetLastError());
 }
 

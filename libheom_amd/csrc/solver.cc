@@ -16,18 +16,6 @@
 namespace hamd {
 
 namespace {
-// Dormand-Prince tableau (rkdp.h:21-57)
-const double DP_A[6][6] = {
-    {0, 0, 0, 0, 0, 0},
-    {1.0 / 5, 0, 0, 0, 0, 0},
-    {3.0 / 40, 9.0 / 40, 0, 0, 0, 0},
-    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0},
-    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0},
-    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0}};
-const double DP_B[6] = {35.0 / 384, 0.0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84};
-const double DP_E[7] = {-71.0 / 57600, 0.0, 71.0 / 16695, -71.0 / 1920,
-                        17253.0 / 339200, -22.0 / 525, 1.0 / 40};
-
 // slot numbers
 enum {
   SL_Y = 0, SL_YNEW = 1, SL_Y32 = 2, SL_YNEW32 = 3,
@@ -223,12 +211,12 @@ void Solver::build_stages(double atol, double rtol) {
       a.k_slot = SL_K1 + i;
       std::vector<std::pair<int, double>> terms{{SL_Y, 1.0}};
       if (i < 5) {
-        for (int j = 0; j < i; ++j) terms.push_back({SL_K1 + j, DP_A[i + 1][j]});
-        a.combo(terms, DP_A[i + 1][i], sout[i]);
+        for (int j = 0; j < i; ++j) terms.push_back({SL_K1 + j, kDpA[i + 1][j]});
+        a.combo(terms, kDpA[i + 1][i], sout[i]);
       } else {
         for (int j = 0; j < 5; ++j)
-          if (DP_B[j] != 0.0) terms.push_back({SL_K1 + j, DP_B[j]});
-        a.combo(terms, DP_B[5], SL_YNEW, lo ? SL_YNEW32 : -1);
+          if (kDpB[j] != 0.0) terms.push_back({SL_K1 + j, kDpB[j]});
+        a.combo(terms, kDpB[5], SL_YNEW, lo ? SL_YNEW32 : -1);
       }
       specs.push_back(a);
     }
@@ -236,8 +224,8 @@ void Solver::build_stages(double atol, double rtol) {
     a.in_slot = lo ? SL_YNEW32 : SL_YNEW;
     a.do_err = 1;
     for (int j = 0; j < 6; ++j)
-      if (DP_E[j] != 0.0) a.ec[a.pre_index(SL_K1 + j)] = DP_E[j];
-    a.ek = DP_E[6];
+      if (kDpE[j] != 0.0) a.ec[a.pre_index(SL_K1 + j)] = kDpE[j];
+    a.ek = kDpE[6];
     // the error weights read y_new / y_old from the last two operand positions
     while (static_cast<int>(a.pre.size()) < kMaxPre - 2) a.pre.push_back(-1);
     a.pre.push_back(SL_YNEW);
@@ -530,6 +518,9 @@ int Solver::solve_gather(void* rho_dev, const double* t_list, int n_t, double dt
                          double rtol, void* rho_s_dev) {
   const size_t rs_bytes = sizeof(z128) * q->N2;
   hipStream_t st = q->stream;
+  if (kind == HEOM_RKDP && n_t >= 2 && use_graphs && in_bytes == 8 && !q->opt.renorm_rho0 &&
+      resident_dopri_ok(q->dev, q->dev.max_links))
+    return solve_resident_dopri(rho_dev, t_list, n_t, dt, atol, rtol, rho_s_dev);
   if (kind == HEOM_RKDP || n_t < 2 || q->kind == kBatch) {
     return solve(rho_dev, t_list, n_t, dt, atol, rtol, [&](int i, const void* cur) {
       HAMD_HIP(hipMemcpyAsync(static_cast<char*>(rho_s_dev) + rs_bytes * i, cur, rs_bytes,
@@ -577,6 +568,50 @@ int Solver::solve_gather(void* rho_dev, const double* t_list, int n_t, double dt
   stats.n_accepted += steps;
   stats.ado_stage_updates = stats.n_rhs * q->hs.n_hrchy;
   return HEOM_OK;
+}
+
+// adaptive_step_size_solver::solve (adaptive_step_size_solver.h:32-56) for a
+// small problem: every interval, attempt and output in one resident launch.
+int Solver::solve_resident_dopri(void* rho_dev, const double* t_list, int n_t, double dt,
+                                 double atol, double rtol, void* rho_s_dev) {
+  for (int i = 0; i + 1 < n_t; ++i)
+    if (!(t_list[i + 1] >= t_list[i]))
+      throw HeomError{HEOM_ERR_ARG, "solve: t_list must be non-decreasing"};
+  if (!(dt > 0.0) && dt_save <= 0.0) throw HeomError{HEOM_ERR_ARG, "rkdp: dt must be > 0"};
+  if (!(atol >= 0.0) || !(rtol >= 0.0) || (atol == 0.0 && rtol == 0.0))
+    throw HeomError{HEOM_ERR_ARG, "rkdp: atol/rtol must be >= 0 and not both 0"};
+  HAMD_HIP(hipSetDevice(q->opt.device));
+  build_stages(atol, rtol);  // kept current for time_stages / stage_bytes
+  reset_graphs();
+  hipStream_t st = q->stream;
+  std::vector<double> t(t_list, t_list + n_t);
+  d_res_t.upload(t);
+  ResidentDopriArgs a{};
+  a.y = static_cast<z128*>(P[SL_Y].p);
+  a.n_int = n_t - 1;
+  a.t_list = d_res_t.as<double>();
+  a.dt_first = dt_save > 0.0 ? dt_save : dt;
+  a.atol = atol;
+  a.rtol = rtol;
+  a.inv_n = 1.0 / static_cast<double>(ms);
+  a.rho_s = static_cast<z128*>(rho_s_dev);
+  a.ctrl = d_ctrl.as<Ctrl>();
+  const size_t bytes = sizeof(z128) * ms;
+  HAMD_HIP(hipMemcpyAsync(P[SL_Y].p, rho_dev, bytes, hipMemcpyDeviceToDevice, st));
+  launch_resident_dopri(q->dev, q->dev.max_links, a, st);
+  HAMD_HIP(hipMemcpyAsync(rho_dev, P[SL_Y].p, bytes, hipMemcpyDeviceToDevice, st));
+  HAMD_HIP(hipMemcpyAsync(h_ctrl, d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+  HAMD_HIP(hipStreamSynchronize(st));
+  dt_save = h_ctrl->dt;
+  stats.n_rhs += 7 * h_ctrl->attempts;
+  stats.n_attempts += h_ctrl->attempts;
+  stats.n_accepted += h_ctrl->accepted;
+  stats.n_rejected += h_ctrl->rejected;
+  stats.last_err = h_ctrl->err;
+  stats.last_dt = dt_save;
+  stats.ado_stage_updates = stats.n_rhs * q->hs.n_hrchy;
+  stats.gave_up |= h_ctrl->gave_up;
+  return h_ctrl->gave_up ? HEOM_ERR_STEP_REJECTED : HEOM_OK;
 }
 
 void Solver::time_stages(int reps, std::vector<double>& out) {

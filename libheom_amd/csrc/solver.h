@@ -108,7 +108,9 @@ class Solver {
   // resident integrator (small heom_hilb problems): stage program built by
   // build_stages when the problem qualifies, else n_stages == 0
   ResidentArgs res{};
-  DevBuf d_res_prog, d_res_nf, d_res_dl;
+  DevBuf d_res_prog, d_res_nf, d_res_dl, d_res_t;
+  int solve_resident_dopri(void* rho_dev, const double* t_list, int n_t, double dt, double atol,
+                           double rtol, void* rho_s_dev);
   // fixed_step_size_solver::solve_1's step sequence over [t0, t1]: n_full steps
   // of dt, then one step of dt_last
   static void fixed_steps(double t0, double t1, double dt, long long& n_full, double& dt_last);

@@ -39,9 +39,11 @@ def test_rk4_graphs_bitwise():
 
 
 def test_rkdp_graphs_bitwise():
-    m = configs.spin_boson(max_depth=6, T=0.5)
+    """Depth 25 (1404 elements): above the resident integrator's size, so both
+    runs take the stage kernels, replayed from graphs or launched one by one."""
+    m = configs.spin_boson(max_depth=25, T=0.5)
     m["t_list"] = np.linspace(0.0, 5.0, 11)
-    q = H.heom_hilb(6).load(m, eng())
+    q = H.heom_hilb(25).load(m, eng())
     (r0, s0, st0), (r1, s1, st1) = both(q, H.rkdp, m, {"dt": 0.8, "atol": 1e-10, "rtol": 1e-8})
     assert st0["n_rejected"] > 0
     assert np.array_equal(r0, r1) and np.array_equal(s0, s1)

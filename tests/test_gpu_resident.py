@@ -1,9 +1,10 @@
-"""Resident fixed-step integrator: small heom_hilb problems (diagonal V, FP64,
-at most 1024 elements, at most 8 links per ADO) run every RK4 / lsrk4 step of
-an output interval in one launch, with the state and stage buffers in LDS.
-It must meet the FP64 bar (1e-10) against the oracle with the oracle's step
-count, shortened last steps included, and agree with the stage-kernel path
-(direct launches, set_graphs(False)) to rounding."""
+"""Resident integrators: small heom_hilb problems (diagonal V, FP64, at most
+1024 elements -- 960 for Dormand-Prince -- and at most 8 links per ADO) run a
+whole solve in one launch with the state and stage buffers in LDS (RK4 / lsrk4
+through heom_solve_dev and per interval otherwise; Dormand-Prince through
+heom_solve_dev with the step control on the device).  They must meet the FP64
+bar (1e-10) against the oracle with the oracle's step / accept / reject counts,
+and agree with the stage-kernel path (set_graphs(False)) to rounding."""
 import numpy as np
 import pytest
 
@@ -85,3 +86,42 @@ def test_largest_resident_problem():
     rho_o, rs_o, st_o, _, _ = oracle(m, "lsrk4", 0.005)
     scale = max(1.0, np.abs(rho_o).max())
     assert np.abs(rho - rho_o).max() <= TOL * scale
+
+
+@pytest.mark.parametrize("case", ["spin_boson_rejections", "eight_links"])
+def test_dopri_resident_matches_oracle(case):
+    if case == "spin_boson_rejections":
+        m = configs.spin_boson(max_depth=6, T=0.5)
+        m["t_list"] = np.linspace(0.0, 5.0, 11)
+        kw = {"dt": 0.8, "atol": 1e-10, "rtol": 1e-8}
+    else:
+        m = configs.random_model(3, 2, 2, 3, seed=9, dense_v=False)
+        m["t_list"] = np.linspace(0.0, 0.5, 6)
+        kw = {"dt": 0.05, "atol": 1e-10, "rtol": 1e-8}
+    rho, rs, st = run(m, "rkdp", kw)
+    o = O.Oracle(O.HILB, m["N"], m["max_depth"], n_threads=8).load(m)
+    rho_o, rs_o, st_o, dts, _ = o.solve(m["rho0"], m["t_list"], O.RKDP, kw["dt"], kw["atol"],
+                                        kw["rtol"])
+    assert (st["n_accepted"], st["n_rejected"]) == (st_o.n_accepted, st_o.n_rejected)
+    assert st["n_rhs"] == 7 * (st_o.n_accepted + st_o.n_rejected)
+    scale = max(1.0, np.abs(rho_o).max())
+    assert np.abs(rho - rho_o).max() <= TOL * scale
+    assert np.abs(rs - rs_o.reshape(rs.shape)).max() <= TOL
+    rho_d, _, st_d = run(m, "rkdp", kw, graphs=False)
+    assert (st_d["n_accepted"], st_d["n_rejected"]) == (st["n_accepted"], st["n_rejected"])
+    assert np.abs(rho - rho_d).max() <= 1e-12 * scale
+
+
+def test_dopri_resident_dt_save_carries_over():
+    """dt_save of one solve seeds the next (adaptive_step_size_solver.h:39-41, 55)."""
+    m = configs.spin_boson(max_depth=4)
+    kw = {"dt": 0.05, "atol": 1e-10, "rtol": 1e-8}
+    g = H.heom_hilb(4).load(m, eng())
+    s = H.qme_solver(eng(), g, H.rkdp())
+    t1, t2 = np.linspace(0.0, 1.0, 3), np.linspace(1.0, 2.0, 3)
+    rho1, _ = s.solve_rho_s(m["rho0"], t1, kw)
+    rho2, _ = s.solve_rho_s(rho1, t2, kw)
+    o = O.Oracle(O.HILB, 2, 4).load(m)
+    r, _, _, dts, _ = o.solve(m["rho0"], t1, O.RKDP, kw["dt"], kw["atol"], kw["rtol"])
+    r, _, _, _, _ = o.solve(r, t2, O.RKDP, kw["dt"], kw["atol"], kw["rtol"], dt_save=dts)
+    assert np.abs(rho2 - r).max() <= TOL

@@ -239,11 +239,15 @@ int heom_solve_dev(heom_solver_handle s, void* rho_dev, const double* t_list, in
 int heom_solver_stats(heom_solver_handle s, heom_stats* out);
 int heom_solver_reset(heom_solver_handle s); /* forget dt_save (adaptive_step_size_solver.h:28) */
 /* 1 (default): replay the fixed stage sequences (RK4 steps of the full dt,
- * Dormand-Prince attempts) from HIP graphs, and run every RK4 / lsrk4 output
- * interval of a small heom_hilb problem (diagonal V, FP64, at most 1024
- * elements, at most 8 links per ADO) in one resident launch with the state in
- * LDS; 0: launch every stage kernel directly.  Results are identical either
- * way.  No counterpart in the reference. */
+ * Dormand-Prince attempts) from HIP graphs, and run a small heom_hilb problem
+ * (diagonal V, FP64, at most 1024 elements -- 960 for Dormand-Prince -- and at
+ * most 8 links per ADO) in resident launches with the state in LDS: one per
+ * RK4 / lsrk4 output interval, or one per heom_solve_dev call (RK4, lsrk4 and
+ * Dormand-Prince with the step control on the device); 0: launch every stage
+ * kernel directly.  Fixed-step results are identical either way; Dormand-Prince
+ * sums its error norm in another order, so a borderline attempt may be decided
+ * differently (the states agree within the tolerance).  No counterpart in the
+ * reference. */
 int heom_solver_set_graphs(heom_solver_handle s, int enable);
 
 /* measurement helpers: average device duration (ms, HIP events on the

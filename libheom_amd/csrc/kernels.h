@@ -39,10 +39,13 @@ void launch_closure_gather(const FormDev& d, const StageArgs& A, float* x, int a
                            hipStream_t stream);
 void launch_gather_rho_s(const FormDev& d, void* dst, const void* src, hipStream_t stream);
 
-// Resident fixed-step integrator (heom_hilb, diagonal V, FP64, at most
-// kResidentMax elements, at most 8 links per ADO, no closure): one launch runs
-// a whole output interval.  resident_ok tells whether a problem qualifies.
+// Resident fixed-step integrator (heom_hilb with diagonal V and at most 8
+// links per ADO, or redfield_hilb with diagonal V; FP64, at most kResidentMax
+// elements, no closure): one launch runs a whole output interval.
+// resident_ok tells whether a problem qualifies.
 bool resident_ok(const FormDev& d, int max_links);
+// LDS bytes of a resident launch with n_buf state-sized buffers
+size_t resident_lds_bytes(const FormDev& d, int n_buf);
 void launch_resident(const FormDev& d, int max_links, const ResidentArgs& a, hipStream_t stream);
 // Resident Dormand-Prince (the fixed-step conditions and at most
 // kResidentDopriMax elements): the whole adaptive solve in one launch.

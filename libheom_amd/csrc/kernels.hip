@@ -1455,6 +1455,12 @@ struct ResidentElem {
     }
   }
 
+  // LDS table: H
+  static __host__ __device__ size_t tab_elems(int N, int n_noise) { return static_cast<size_t>(N) * N; }
+  static __device__ __forceinline__ void stage(const FormDev& d, z128* tab) {
+    for (int x = threadIdx.x; x < d.N2; x += blockDim.x) tab[x] = d.H[x];
+  }
+
   // D at element e of the state X (same expression order as hilb_fast_kernel)
   __device__ __forceinline__ z128 op(const z128* X, const z128* s_H, int e, int N, int N2) const {
     const z128* Xg = X + static_cast<size_t>(g) * N2;
@@ -1473,7 +1479,62 @@ struct ResidentElem {
   }
 };
 
-template <int MAXL, int SCHEME>
+// One element of a resident redfield_hilb problem with diagonal V: the
+// operator of redfield_small_kernel<..., true> (D = HL rho + rho HR -
+// sum_u (v_u[i] (rho Lambda_u^+)[i][j] + (Lambda_u rho)[i][j] v_u[j])), same
+// expression order, with its tables in LDS.
+struct RedfieldElem {
+  int i, j, nn;
+  // LDS table: HL, HR, per noise Lambda_u and Lambda_u^+, then v_u (n_noise x N)
+  static __host__ __device__ size_t tab_elems(int N, int n_noise) {
+    return static_cast<size_t>(2 + 2 * n_noise) * N * N + static_cast<size_t>(n_noise) * N;
+  }
+  static __device__ __forceinline__ void stage(const FormDev& d, z128* tab) {
+    const int N2 = d.N2, nn = d.n_noise;
+    for (int x = threadIdx.x; x < N2; x += blockDim.x) {
+      tab[x] = d.rf_HL[x];
+      tab[N2 + x] = d.rf_HR[x];
+    }
+    for (int x = threadIdx.x; x < nn * N2; x += blockDim.x) {
+      const int u = x / N2, y = x - u * N2;
+      tab[static_cast<size_t>(2 + 2 * u) * N2 + y] = d.rf_L[x];
+      tab[static_cast<size_t>(3 + 2 * u) * N2 + y] = d.rf_Ld[x];
+    }
+    for (int x = threadIdx.x; x < nn * d.N; x += blockDim.x)
+      tab[static_cast<size_t>(2 + 2 * nn) * N2 + x] = d.Vd[x];
+  }
+  __device__ __forceinline__ void init(const FormDev& d, int e, bool ok) {
+    const int ee = ok ? e : 0;
+    i = ee / d.N;
+    j = ee - i * d.N;
+    nn = d.n_noise;
+  }
+  __device__ __forceinline__ z128 op(const z128* X, const z128* tab, int e, int N, int N2) const {
+    const z128* HL = tab;
+    const z128* HR = tab + N2;
+    z128 D{0.0, 0.0};
+    for (int k = 0; k < N; ++k) {
+      cmac(D, HL[i * N + k], X[k * N + j]);
+      cmac(D, X[i * N + k], HR[k * N + j]);
+    }
+    const z128* v = tab + static_cast<size_t>(2 + 2 * nn) * N2;
+    for (int u = 0; u < nn; ++u) {
+      const z128* L = tab + static_cast<size_t>(2 + 2 * u) * N2;
+      const z128* Ld = L + N2;
+      z128 x{0.0, 0.0}, y{0.0, 0.0};
+      for (int k = 0; k < N; ++k) {
+        cmac(x, X[i * N + k], Ld[k * N + j]);
+        cmac(y, L[i * N + k], X[k * N + j]);
+      }
+      const z128 vi = v[u * N + i], vj = v[u * N + j];
+      cmac(D, z128{-vi.re, -vi.im}, x);
+      cmac(D, z128{-vj.re, -vj.im}, y);
+    }
+    return D;
+  }
+};
+
+template <class E, int SCHEME>
 __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const ResidentArgs a) {
   extern __shared__ __align__(16) unsigned char res_lds[];
   z128* buf = reinterpret_cast<z128*>(res_lds);
@@ -1481,13 +1542,13 @@ __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const R
   const ResidentStage* __restrict__ prog = a.prog;
   const int ms = static_cast<int>(d.main_size);
   const int N = d.N, N2 = d.N2;
-  z128* s_H = buf + static_cast<size_t>(a.n_buf) * ms;
+  z128* s_H = buf + static_cast<size_t>(a.n_buf) * ms;  // the element type's tables
   const int e = threadIdx.x;
   const bool ok = e < ms;
   for (int x = e; x < a.n_buf * ms; x += blockDim.x)
     buf[x] = x < ms ? a.y[x] : z128{0.0, 0.0};
-  for (int x = e; x < N2; x += blockDim.x) s_H[x] = d.H[x];
-  ResidentElem<MAXL> el;
+  E::stage(d, s_H);
+  E el;
   el.init(d, e, ok);
   __syncthreads();
   const int n_int = a.n_int > 0 ? a.n_int : 1;
@@ -1543,7 +1604,7 @@ __global__ void __launch_bounds__(1024) resident_kernel(const FormDev d, const R
 // ctrl_init_kernel does.  The error norm is summed in a fixed order (wavefront
 // shuffles, then the waves in order).  LDS buffers: 0/1 the y / y_new pair
 // (swapped by Ctrl::cur on acceptance), 2..7 k1..k6, 8/9 the stage inputs.
-template <int MAXL>
+template <class E>
 __global__ void __launch_bounds__(1024)
     resident_dopri_kernel(const FormDev d, const ResidentDopriArgs a) {
   extern __shared__ __align__(16) unsigned char res_lds[];
@@ -1558,8 +1619,8 @@ __global__ void __launch_bounds__(1024)
   const bool ok = e < ms;
   for (int x = e; x < kResidentDopriBuf * ms; x += blockDim.x)
     buf[x] = x < ms ? a.y[x] : z128{0.0, 0.0};
-  for (int x = e; x < N2; x += blockDim.x) s_H[x] = d.H[x];
-  ResidentElem<MAXL> el;
+  E::stage(d, s_H);
+  E el;
   el.init(d, e, ok);
   if (e == 0) {
     s_c = Ctrl{};
@@ -1964,43 +2025,58 @@ void launch_rho0_renorm(void* y0, void* y1, void* f0, void* f1, const Ctrl* ctrl
 }
 
 bool resident_ok(const FormDev& d, int max_links) {
-  return d.kind == HEOM_HILB && d.diag_v && d.main_size <= kResidentMax && max_links <= 8 &&
-         d.clos == nullptr;
+  if (d.main_size > kResidentMax || d.clos != nullptr) return false;
+  if (d.kind == HEOM_HILB) return d.diag_v && max_links <= 8;
+  return d.kind == HEOM_REDFIELD_HILB && d.rf_HL != nullptr;  // diagonal V
+}
+
+namespace {
+size_t resident_tab_elems(const FormDev& d) {
+  return d.kind == HEOM_REDFIELD_HILB ? RedfieldElem::tab_elems(d.N, d.n_noise)
+                                      : ResidentElem<4>::tab_elems(d.N, d.n_noise);
+}
+}  // namespace
+
+size_t resident_lds_bytes(const FormDev& d, int n_buf) {
+  return sizeof(z128) * (static_cast<size_t>(n_buf) * d.main_size + resident_tab_elems(d));
 }
 
 void launch_resident(const FormDev& d, int max_links, const ResidentArgs& a, hipStream_t stream) {
   const int ms = static_cast<int>(d.main_size);
   if (!resident_ok(d, max_links) || a.n_stages < 1 || a.n_stages > 4 || a.n_buf < 1 || !a.prog)
     throw HeomError{HEOM_ERR_STATE, "resident integrator: unsupported problem"};
-  const size_t lds = sizeof(z128) * (static_cast<size_t>(a.n_buf) * ms + d.N2);
+  const size_t lds = resident_lds_bytes(d, a.n_buf);
   if (lds > kResidentLds) throw HeomError{HEOM_ERR_STATE, "resident integrator: LDS exceeded"};
   const int bs = ((ms + 63) / 64) * 64;
-#define HAMD_RES(L, S)                                                                \
+#define HAMD_RES(E, S)                                                                \
   do {                                                                                \
-    static bool once = (allow_lds(resident_kernel<L, S>, kResidentLds), true);        \
+    static bool once = (allow_lds(resident_kernel<E, S>, kResidentLds), true);        \
     (void)once;                                                                       \
-    hipLaunchKernelGGL((resident_kernel<L, S>), dim3(1), dim3(bs), lds, stream, d, a); \
+    hipLaunchKernelGGL((resident_kernel<E, S>), dim3(1), dim3(bs), lds, stream, d, a); \
   } while (0)
-#define HAMD_RES_S(L)       \
+#define HAMD_RES_S(E)       \
   do {                      \
     if (a.scheme == 1)      \
-      HAMD_RES(L, 1);       \
+      HAMD_RES(E, 1);       \
     else if (a.scheme == 2) \
-      HAMD_RES(L, 2);       \
+      HAMD_RES(E, 2);       \
     else                    \
-      HAMD_RES(L, 0);       \
+      HAMD_RES(E, 0);       \
   } while (0)
-  if (max_links <= 4)
-    HAMD_RES_S(4);
+  if (d.kind == HEOM_REDFIELD_HILB)
+    HAMD_RES_S(RedfieldElem);
+  else if (max_links <= 4)
+    HAMD_RES_S(ResidentElem<4>);
   else
-    HAMD_RES_S(8);
+    HAMD_RES_S(ResidentElem<8>);
 #undef HAMD_RES_S
 #undef HAMD_RES
   HAMD_HIP(hipGetLastError());
 }
 
 bool resident_dopri_ok(const FormDev& d, int max_links) {
-  return resident_ok(d, max_links) && d.main_size <= kResidentDopriMax;
+  return resident_ok(d, max_links) && d.main_size <= kResidentDopriMax &&
+         resident_lds_bytes(d, kResidentDopriBuf) <= kResidentLds;
 }
 
 void launch_resident_dopri(const FormDev& d, int max_links, const ResidentDopriArgs& a,
@@ -2008,19 +2084,21 @@ void launch_resident_dopri(const FormDev& d, int max_links, const ResidentDopriA
   if (!resident_dopri_ok(d, max_links) || a.n_int < 1)
     throw HeomError{HEOM_ERR_STATE, "resident Dormand-Prince: unsupported problem"};
   const int ms = static_cast<int>(d.main_size);
-  const size_t lds = sizeof(z128) * (static_cast<size_t>(kResidentDopriBuf) * ms + d.N2);
+  const size_t lds = resident_lds_bytes(d, kResidentDopriBuf);
   if (lds > kResidentLds) throw HeomError{HEOM_ERR_STATE, "resident Dormand-Prince: LDS exceeded"};
   const int bs = ((ms + 63) / 64) * 64;
-#define HAMD_RDP(L)                                                                      \
+#define HAMD_RDP(E)                                                                      \
   do {                                                                                   \
-    static bool once = (allow_lds(resident_dopri_kernel<L>, kResidentLds), true);        \
+    static bool once = (allow_lds(resident_dopri_kernel<E>, kResidentLds), true);        \
     (void)once;                                                                          \
-    hipLaunchKernelGGL((resident_dopri_kernel<L>), dim3(1), dim3(bs), lds, stream, d, a); \
+    hipLaunchKernelGGL((resident_dopri_kernel<E>), dim3(1), dim3(bs), lds, stream, d, a); \
   } while (0)
-  if (max_links <= 4)
-    HAMD_RDP(4);
+  if (d.kind == HEOM_REDFIELD_HILB)
+    HAMD_RDP(RedfieldElem);
+  else if (max_links <= 4)
+    HAMD_RDP(ResidentElem<4>);
   else
-    HAMD_RDP(8);
+    HAMD_RDP(ResidentElem<8>);
 #undef HAMD_RDP
   HAMD_HIP(hipGetLastError());
 }

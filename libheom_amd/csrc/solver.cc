@@ -279,8 +279,7 @@ void Solver::build_resident() {
     }
   }
   r.n_buf = static_cast<int>(bmap.size());
-  const size_t lds = sizeof(z128) * (static_cast<size_t>(r.n_buf) * ms + q->N2);
-  if (lds > kResidentLds) return;
+  if (resident_lds_bytes(q->dev, r.n_buf) > kResidentLds) return;
   std::vector<ResidentStage> prog(r.st, r.st + r.n_stages);
   d_res_prog.upload(prog);
   r.prog = d_res_prog.as<ResidentStage>();

@@ -415,7 +415,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not provided "
-              "(see DESIGN.md for how the hierarchy would shard)", file=sys.stderr)
+              "(the hierarchy is not sharded across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import libheom_amd as H

@@ -241,7 +241,8 @@ int heom_solver_reset(heom_solver_handle s); /* forget dt_save (adaptive_step_si
 /* 1 (default): replay the fixed stage sequences (RK4 steps of the full dt,
  * Dormand-Prince attempts) from HIP graphs, and run a small heom_hilb problem
  * (diagonal V, FP64, at most 1024 elements -- 960 for Dormand-Prince -- and at
- * most 8 links per ADO) in resident launches with the state in LDS: one per
+ * most 8 links per ADO) or a small redfield_hilb problem (diagonal V, FP64,
+ * its tables within LDS) in resident launches with the state in LDS: one per
  * RK4 / lsrk4 output interval, or one per heom_solve_dev call (RK4, lsrk4 and
  * Dormand-Prince with the step control on the device); 0: launch every stage
  * kernel directly.  Fixed-step results are identical either way; Dormand-Prince

@@ -677,6 +677,34 @@ __device__ __forceinline__ mf_f4 mac3(const Split8& a, const Split8& b, mf_f4 c)
 
 constexpr int kMfmaBS = 256;
 
+// The link coefficient table entries of element tb of one noise: (cm, ac) as
+// two reals for real V, else the complex pair.
+__device__ __forceinline__ void link_coef_load(const FormDev& d, long long tb, z128& p0,
+                                               z128& p1) {
+  if (d.real_v) {
+    p0 = d.cmac_tab[tb];
+  } else {
+    p0 = d.cmac_tab[2 * tb];
+    p1 = d.cmac_tab[2 * tb + 1];
+  }
+}
+
+// Coefficient of a neighbour element: cm*iphi + ac*psi + dir (diagonal V).
+template <typename R>
+__device__ __forceinline__ cx<R> link_coef(const FormDev& d, const LinkFlat& q, const z128& p0,
+                                           const z128& p1) {
+  cx<R> cf;
+  if (d.real_v) {
+    cf.re = static_cast<R>(p0.re) * static_cast<R>(q.iphi.re) +
+            static_cast<R>(p0.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
+    cf.im = static_cast<R>(p0.re) * static_cast<R>(q.iphi.im) +
+            static_cast<R>(p0.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
+  } else {
+    cf = conv<R>(p0) * conv<R>(q.iphi) + conv<R>(p1) * conv<R>(q.psi) + conv<R>(q.dir);
+  }
+  return cf;
+}
+
 template <int N>
 struct MfmaCfg {
   static constexpr int N2 = N * N;
@@ -784,14 +812,14 @@ __device__ __forceinline__ void mfma_commutator_bf16x3(const FormDev& d, const f
   }
 }
 
-// FP32 operands (complex64, BF16X3) ask for two waves per SIMD (measured 17% /
-// 5% faster at N=64); FP64 keeps one (the bound costs it 3%).
+// Every form asks for two waves per SIMD (the register budget of 256 per lane;
+// LDS holds two FP64 workgroups per CU).
 // BF16X3: R = float, products via mfma_commutator_bf16x3.  N is the tile size
 // (32 or 64); the ADO block is n x n with n = d.N in (N/2, N] for N = 64, in
 // (16, 32] for N = 32, zero-padded in LDS (and H in its planes), so that the
 // padded rows and columns of the products are exactly zero.
 template <int N, typename R, bool REALH, bool F64, int KP, bool BF16X3, bool PADDED>
-__global__ void __launch_bounds__(kMfmaBS, sizeof(R) == 4 ? 2 : 1)
+__global__ void __launch_bounds__(kMfmaBS, 2)
     hilb_mfma_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   using M = Mfma<R>;
@@ -870,12 +898,53 @@ __global__ void __launch_bounds__(kMfmaBS, sizeof(R) == 4 ? 2 : 1)
       }
     }
     __syncthreads();
-    // neighbour couplings and the Runge-Kutta epilogue, element-parallel and
-    // coalesced, CH elements per thread and pass: every load of a pass is
-    // issued before its stores (the stores may alias later loads)
-    // (chunk elements past the block read element 0 and store nothing)
+    // Neighbour couplings and the Runge-Kutta epilogue, element-parallel and
+    // coalesced.  Two schedules of the same per-element arithmetic (D, then
+    // the links in order, then the epilogue):
+    //  - links outer (FP64 and BF16X3 forms): for each link the loads of LCH of
+    //    the thread's elements of the neighbour block are in flight together;
+    //    each thread updates only its own elements (x = tid + c*BS) in LDS and
+    //    reads them back in the epilogue, so the phases need no barrier;
+    //  - links inside the epilogue passes (complex64 form, whose register
+    //    budget keeps three waves per SIMD that way).
+    // Each epilogue pass takes CH elements per thread and issues every load
+    // before its stores (the stores may alias later loads); elements past the
+    // block read element 0 and store nothing.
+    constexpr bool kLinksOuter = sizeof(R) == 8 || BF16X3;
+    constexpr int EPT = N2 / kMfmaBS;         // elements per thread (tile)
+    constexpr int LCH = EPT < 8 ? EPT : 8;    // elements per load group
     constexpr int CH = KP <= 4 ? 2 : 1;
     const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
+    if constexpr (kLinksOuter) {
+      for (int l = l0; l < l1; ++l) {
+        const LinkFlat q = ld_flat(flat + l);
+        const C* nb = in + static_cast<long long>(q.nbr) * n2;
+        const long long tu = static_cast<long long>(q.u) * n2;
+#pragma unroll 1
+        for (int c0 = 0; c0 < EPT; c0 += LCH) {
+          C xv[LCH];
+          z128 p0[LCH], p1[LCH];
+#pragma unroll
+          for (int c = 0; c < LCH; ++c) {
+            const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+            const int xs = x < n2 ? x : 0;
+            xv[c] = nb[xs];
+            link_coef_load(d, tu + xs, p0[c], p1[c]);
+          }
+#pragma unroll
+          for (int c = 0; c < LCH; ++c) {
+            const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+            if (x < n2) {
+              const int i = x / n, j = x - (x / n) * n;
+              C Dv{s_re[i * LD + j], s_im[i * LD + j]};
+              cmac(Dv, link_coef<R>(d, q, p0[c], p1[c]), xv[c]);
+              s_re[i * LD + j] = Dv.re;
+              s_im[i * LD + j] = Dv.im;
+            }
+          }
+        }
+      }
+    }
     for (int x0 = threadIdx.x; x0 < n2; x0 += kMfmaBS * CH) {
       C Dc[CH];
       PreT<KP> pv[CH];
@@ -888,34 +957,18 @@ __global__ void __launch_bounds__(kMfmaBS, sizeof(R) == 4 ? 2 : 1)
         Dc[c] = C{s_re[i * LD + j], s_im[i * LD + j]};
         prefetch<F64, KP>(pv[c], A, cur, base + xs[c], true);
       }
-      for (int l = l0; l < l1; ++l) {
-        const LinkFlat q = ld_flat(flat + l);
-        C xv[CH];
-        z128 p0[CH], p1[CH];
+      if constexpr (!kLinksOuter) {
+        for (int l = l0; l < l1; ++l) {
+          const LinkFlat q = ld_flat(flat + l);
+          C xv[CH];
+          z128 p0[CH], p1[CH];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          xv[c] = in[static_cast<long long>(q.nbr) * n2 + xs[c]];
-          const long long tb = static_cast<long long>(q.u) * n2 + xs[c];
-          if (d.real_v) {
-            p0[c] = d.cmac_tab[tb];
-          } else {
-            p0[c] = d.cmac_tab[2 * tb];
-            p1[c] = d.cmac_tab[2 * tb + 1];
+          for (int c = 0; c < CH; ++c) {
+            xv[c] = in[static_cast<long long>(q.nbr) * n2 + xs[c]];
+            link_coef_load(d, static_cast<long long>(q.u) * n2 + xs[c], p0[c], p1[c]);
           }
-        }
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          C cf;
-          if (d.real_v) {
-            cf.re = static_cast<R>(p0[c].re) * static_cast<R>(q.iphi.re) +
-                    static_cast<R>(p0[c].im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
-            cf.im = static_cast<R>(p0[c].re) * static_cast<R>(q.iphi.im) +
-                    static_cast<R>(p0[c].im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
-          } else {
-            cf = conv<R>(p0[c]) * conv<R>(q.iphi) + conv<R>(p1[c]) * conv<R>(q.psi) +
-                 conv<R>(q.dir);
-          }
-          cmac(Dc[c], cf, xv[c]);
+          for (int c = 0; c < CH; ++c) cmac(Dc[c], link_coef<R>(d, q, p0[c], p1[c]), xv[c]);
         }
       }
 #pragma unroll

@@ -1,10 +1,11 @@
 """Resident integrators: small heom_hilb problems (diagonal V, FP64, at most
-1024 elements -- 960 for Dormand-Prince -- and at most 8 links per ADO) run a
-whole solve in one launch with the state and stage buffers in LDS (RK4 / lsrk4
-through heom_solve_dev and per interval otherwise; Dormand-Prince through
-heom_solve_dev with the step control on the device).  They must meet the FP64
-bar (1e-10) against the oracle with the oracle's step / accept / reject counts,
-and agree with the stage-kernel path (set_graphs(False)) to rounding."""
+1024 elements -- 960 for Dormand-Prince -- and at most 8 links per ADO) and
+small redfield_hilb problems (diagonal V) run a whole solve in one launch with
+the state and stage buffers in LDS (RK4 / lsrk4 through heom_solve_dev and per
+interval otherwise; Dormand-Prince through heom_solve_dev with the step control
+on the device).  They must meet the FP64 bar (1e-10) against the oracle with the
+oracle's step / accept / reject counts, and agree with the stage-kernel path
+(set_graphs(False)) to rounding."""
 import numpy as np
 import pytest
 
@@ -125,3 +126,32 @@ def test_dopri_resident_dt_save_carries_over():
     r, _, _, dts, _ = o.solve(m["rho0"], t1, O.RKDP, kw["dt"], kw["atol"], kw["rtol"])
     r, _, _, _, _ = o.solve(r, t2, O.RKDP, kw["dt"], kw["atol"], kw["rtol"], dt_save=dts)
     assert np.abs(rho2 - r).max() <= TOL
+
+
+@pytest.mark.parametrize("solver", ["rk4", "lsrk4", "rkdp"])
+def test_redfield_hilb_resident_against_stage_kernels_and_oracle(solver):
+    """Redfield 7-site chain (7 site baths, diagonal V) to 100 fs, rho_S every 10 fs:
+    the resident integrator (default) and the stage kernels (set_graphs(False))
+    both against the oracle, with its step / accept / reject counts."""
+    m = configs.exciton_chain7(max_depth=0, t_final=100.0)
+    m["rho0"] = m["rho0"][:49].copy()
+    kw = ({"dt": m["dt"], "atol": m["atol"], "rtol": m["rtol"]} if solver == "rkdp"
+          else {"dt": 0.1})
+    out = {}
+    for graphs in (True, False):
+        g = H.redfield_hilb().load(m, eng())
+        s = H.qme_solver(eng(), g, getattr(H, solver)())
+        s.set_graphs(graphs)
+        rho, rs = s.solve_rho_s(m["rho0"], m["t_list"], kw)
+        out[graphs] = (rho, rs, s.stats())
+    o = O.Oracle(O.REDFIELD_HILB, 7, 0, n_threads=8).load(m)
+    rho_o, rs_o, st_o, _, _ = o.solve(m["rho0"], m["t_list"], getattr(O, solver.upper()),
+                                      kw["dt"], kw.get("atol", 0.0), kw.get("rtol", 0.0))
+    for rho, rs, st in out.values():
+        assert np.abs(rs - rs_o).max() <= TOL
+        assert np.abs(rho - rho_o).max() <= TOL
+        if solver == "rkdp":
+            assert (st["n_accepted"], st["n_rejected"]) == (st_o.n_accepted, st_o.n_rejected)
+        else:
+            assert st["n_rhs"] == st_o.n_rhs
+    assert np.abs(out[True][0] - out[False][0]).max() <= 1e-12

@@ -59,14 +59,14 @@ typedef struct heom_solver* heom_solver_handle;
 
 typedef struct heom_options {
   int parity_q1;  /* 1 (default): drop off-diagonal gamma like the reference
-                     (heom.h:190-205 clears gamma before copying it) */
+                     (heom.h:106-121 clears gamma at set_shape, line 110, before copying it) */
   int sqrt_norm;  /* 1: LIBHEOM_SQRT_NORMALIZATION factors (heom_hilb.h:140-196) */
   int order;      /* 0: row_major (default), 1: col_major (matrix_base.h) */
   int device;     /* HIP device ordinal */
   int matrix_cores; /* 1 (default): heom_hilb with 16 < n_level <= 64 and diagonal
                        V computes the commutator on MFMA (blocks zero-padded to 32
                        or 64 rows); 0: vector ALU kernels */
-  int renorm_rho0;  /* 1: after every RK step / Dormand-Prince attempt, replace rho_0
+  int renorm_rho0;  /* 1: after every RK step / accepted Dormand-Prince step, replace rho_0
                        (rho_S) by (rho_0 + rho_0^+)/2 scaled to the trace it had when
                        the solve started, in FP64 on the device (n_level <= 64, not
                        for batches).  0 (default, parity mode): off.  No counterpart
@@ -219,13 +219,20 @@ int heom_closure_rows(heom_handle deep, heom_handle shallow, const void* rho_dee
                       const float* features, int n_features, int layout, float* x_dev,
                       float* y_dev);
 
-/* solver_base::init via qme_solver (qme_solver.h:26-35) */
+/* solver_base::init via qme_solver (qme_solver.h:26-35).  The solver shares
+ * ownership of the QME: heom_destroy(h) leaves it usable until
+ * heom_solver_destroy.  If set_param is called again on h, the solver follows
+ * the new parameters; a solve returns HEOM_ERR_STATE when h is not set up or
+ * its main_size differs from the one the solver was created for. */
 int heom_solver_create(heom_handle h, int solver_kind, heom_solver_handle* out);
 int heom_solver_destroy(heom_solver_handle s);
 
 /* qme_solver::solve / solver_base::solve (solver_base.h:64-79): rho_host
  * (main_size complex128) is propagated in place over t_list; kwargs dt, atol,
  * rtol as in fixed_step_size_solver.h:37 / adaptive_step_size_solver.h:40-43.
+ * t_list must be non-decreasing; under Dormand-Prince a zero-length interval
+ * takes no attempt and keeps the step size (the reference would clamp the step
+ * to 0 there and never finish the next interval).
  * full_copy = 1 copies the whole state to rho_host at every output time
  * (reference behaviour, solver_base.h:74-76); 0 copies rho_S per output and the
  * whole state only at the end. */
@@ -265,6 +272,11 @@ int heom_dev_free(void* ptr);
 int heom_memcpy_h2d(void* dst_dev, const void* src_host, long long bytes);
 int heom_memcpy_d2h(void* dst_host, const void* src_dev, long long bytes);
 int heom_device_sync(void);
+/* Measurement helper, no counterpart in the reference: a grid-stride copy of
+ * `bytes` between two fresh device buffers (grid x 256 threads, 16-byte
+ * elements), timed with HIP events over `reps` launches after one warm-up;
+ * *gbs = (bytes read + bytes written) per second / 1e9. */
+int heom_stream_copy(long long bytes, int grid, int reps, double* gbs);
 
 #ifdef __cplusplus
 }

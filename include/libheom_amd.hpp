@@ -22,6 +22,7 @@
 
 #include <any>
 #include <complex>
+#include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
@@ -143,10 +144,6 @@ class qme_base {
   virtual int temp_size() { return 0; }
 
   virtual void set_param(engine* obj) {
-    if (h_) {
-      heom_destroy(h_);
-      h_ = nullptr;
-    }
     heom_options opt;
     heom_default_options(&opt);
     opt.order = order == row_major ? 0 : 1;
@@ -156,7 +153,19 @@ class qme_base {
     opt.matrix_cores = matrix_cores;
     opt.renorm_rho0 = renorm_rho0;
     const int dt = std::is_same<dtype, complex64>::value ? HEOM_C64 : HEOM_C128;
-    check(heom_create(kind_, dt, n_level, max_depth_, &opt, &h_));
+    // the handle is set up again in place while its create-time options are
+    // unchanged, so a qme_solver built on this object sees the new parameters
+    // (the reference's qme_solver keeps a pointer to the same qme)
+    if (h_ && (n_level != h_n_level_ || std::memcmp(&opt, &h_opt_, sizeof(opt)) != 0)) {
+      heom_destroy(h_);  // solvers bound to it keep it alive and rebind on their next solve
+      h_ = nullptr;
+    }
+    if (!h_) {
+      check(heom_create(kind_, dt, n_level, max_depth_, &opt, &h_));
+      h_opt_ = opt;
+      h_n_level_ = n_level;
+    }
+    check(heom_set_hrchy_filter(h_, nullptr, nullptr));
     check(heom_set_H(h_, H.dense(n_level, n_level).data()));
     check(heom_alloc_noises(h_, n_noise));
     for (int u = 0; u < n_noise; ++u) {
@@ -206,6 +215,8 @@ class qme_base {
   int max_depth_ = 0;
   int n_hrchy_ = 0;
   heom_handle h_ = nullptr;
+  heom_options h_opt_{};  // create-time options of h_
+  int h_n_level_ = 0;
 };
 
 template <typename dtype, order_t order, typename engine>
@@ -324,6 +335,7 @@ class qme_solver {
              solver_base<dtype, order, engine>* solver_)
       : eng(engine_), qme(qme_), solver(solver_) {
     check(heom_solver_create(qme->handle(), solver->kind(), &s_));
+    qh_ = qme->handle();
   }
   ~qme_solver() {
     if (s_) heom_solver_destroy(s_);
@@ -380,6 +392,13 @@ class qme_solver {
       (*c->sync)();
       if (*c->cb) (*c->cb)(static_cast<real_t<dtype>>(t));
     };
+    if (qme->handle() != qh_) {  // set_param replaced the qme's handle: rebind
+      heom_solver_handle s2 = nullptr;
+      check(heom_solver_create(qme->handle(), solver->kind(), &s2));
+      heom_solver_destroy(s_);
+      s_ = s2;
+      qh_ = qme->handle();
+    }
     int rc = heom_solve(s_, buf.data(), ts.data(), n_t, trampoline, &ctx, dt, atol, rtol, 1);
     from_buf();
     check(rc);
@@ -393,6 +412,7 @@ class qme_solver {
 
  private:
   heom_solver_handle s_ = nullptr;
+  heom_handle qh_ = nullptr;  // the qme handle s_ was created on
 };
 
 }  // namespace libheom

@@ -113,19 +113,29 @@ class _heom:
         if self.n_level <= 0:
             raise HeomError(_capi.HEOM_ERR_ARG, "n_level not set")
         N = self.n_level
-        if self._h:
-            L.heom_destroy(self._h)
-            self._h = None
         opt = _capi.heom_options(int(self.parity_q1), int(self.sqrt_normalization),
                                  0 if self.order == row_major else 1, dev,
                                  int(bool(self.matrix_cores)), int(bool(self.renorm_rho0)))
-        h = ctypes.c_void_p()
         dt = _capi.HEOM_C64 if self.dtype == np.complex64 else _capi.HEOM_C128
         if self.bf16x3:
             dt = _capi.HEOM_C64_BF16X3
-        _capi.check(L.heom_create(self._kind, dt, N, self.max_depth, ctypes.byref(opt),
-                                  ctypes.byref(h)))
-        self._h = h
+        key = (self._kind, dt, N, self.max_depth) + tuple(getattr(opt, f) for f, _ in
+                                                          opt._fields_)
+        if self._h and key != getattr(self, "_hkey", None):
+            # create-time options changed: a new handle (solvers bound to the old
+            # one keep it alive and rebind on their next solve)
+            L.heom_destroy(self._h)
+            self._h = None
+        if not self._h:
+            h = ctypes.c_void_p()
+            _capi.check(L.heom_create(self._kind, dt, N, self.max_depth, ctypes.byref(opt),
+                                      ctypes.byref(h)))
+            self._h = h
+            self._hkey = key
+        # the same handle is set up again in place, so solvers created on this
+        # qme see the new parameters (qme_solver holds the qme, qme_solver.h:18-53)
+        h = self._h
+        self._ready = False
         if self.H is None:
             raise HeomError(_capi.HEOM_ERR_ARG, "H not set")
         H = _c128(self.H, (N, N))
@@ -159,6 +169,9 @@ class _heom:
 
             self._filter_cb = _capi.FILTER(_filter)
             _capi.check(L.heom_set_hrchy_filter(h, self._filter_cb, None))
+        else:
+            _capi.check(L.heom_set_hrchy_filter(h, _capi.FILTER(), None))
+            self._filter_cb = None
         _capi.check(L.heom_set_param(h))
         if errors:
             raise errors[0]
@@ -377,9 +390,28 @@ class qme_solver:
         if not qme._ready:
             raise HeomError(_capi.HEOM_ERR_STATE, "set_param has not been called")
         self.engine, self.qme, self.solver = engine, qme, solver
+        self._graphs = True
+        self._bind()
+
+    def _bind(self):
         s = ctypes.c_void_p()
-        _capi.check(_capi.load().heom_solver_create(qme._h, solver._kind, ctypes.byref(s)))
+        _capi.check(_capi.load().heom_solver_create(self.qme._h, self.solver._kind,
+                                                    ctypes.byref(s)))
+        if getattr(self, "_s", None):
+            _capi.load().heom_solver_destroy(self._s)
         self._s = s
+        self._qh = self.qme._h.value
+        if not self._graphs:
+            _capi.check(_capi.load().heom_solver_set_graphs(self._s, 0))
+
+    def _sync(self):
+        """Rebind when the qme's set_param replaced its handle (changed dtype,
+        order, device or options); a qme set up again in place keeps the solver,
+        whose buffers must still fit its main_size (else HEOM_ERR_STATE)."""
+        if not self.qme._ready:
+            raise HeomError(_capi.HEOM_ERR_STATE, "set_param has not been called")
+        if self.qme._h.value != self._qh:
+            self._bind()
 
     def __del__(self):
         if getattr(self, "_s", None):
@@ -402,6 +434,7 @@ class qme_solver:
 
     def solve(self, rho, t_list, callback=None, kwargs=None, full_copy=True):
         """solver_base::solve (solver_base.h:64-79): rho is propagated in place."""
+        self._sync()
         dt, atol, rtol = self._kw(kwargs or {})
         if not (isinstance(rho, np.ndarray) and rho.dtype == np.complex128 and
                 rho.flags.c_contiguous):
@@ -431,6 +464,7 @@ class qme_solver:
 
         Device-resident path: rho_S is gathered in HBM and copied once at the end.
         """
+        self._sync()
         dt, atol, rtol = self._kw(kwargs)
         L = _capi.load()
         rho = np.array(rho, dtype=np.complex128, copy=True).reshape(-1)
@@ -462,4 +496,5 @@ class qme_solver:
 
     def set_graphs(self, enable):
         """Replay fixed stage sequences from HIP graphs (default) or launch directly."""
+        self._graphs = bool(enable)
         _capi.check(_capi.load().heom_solver_set_graphs(self._s, int(bool(enable))))

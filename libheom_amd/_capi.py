@@ -32,7 +32,7 @@ EXPORTS = [
     "heom_calc_diff_host", "heom_batch_create", "heom_batch_layout", "heom_solver_create", "heom_solver_destroy", "heom_solve",
     "heom_solve_dev", "heom_solver_stats", "heom_solver_reset", "heom_solver_set_graphs", "heom_solver_time_stages",
     "heom_solver_stage_bytes", "heom_dev_alloc", "heom_dev_free", "heom_memcpy_h2d",
-    "heom_memcpy_d2h", "heom_device_sync", "heom_mlp_create", "heom_mlp_destroy",
+    "heom_memcpy_d2h", "heom_device_sync", "heom_stream_copy", "heom_mlp_create", "heom_mlp_destroy",
     "heom_mlp_n_params", "heom_mlp_get_params", "heom_mlp_set_params", "heom_mlp_set_scaling",
     "heom_mlp_get_scaling", "heom_mlp_fit_scaling", "heom_mlp_forward", "heom_mlp_loss_grad",
     "heom_mlp_adam_step", "heom_mlp_train", "heom_set_closure", "heom_closure_dims",
@@ -116,6 +116,7 @@ def load():
         "heom_memcpy_h2d": (ip, [vp, vp, llp]),
         "heom_memcpy_d2h": (ip, [vp, vp, llp]),
         "heom_device_sync": (ip, []),
+        "heom_stream_copy": (ip, [llp, ip, ip, P(dp)]),
         "heom_mlp_create": (ip, [ip, ip, ip, ctypes.c_ulonglong, ip, P(vp)]),
         "heom_mlp_destroy": (ip, [vp]),
         "heom_mlp_n_params": (ip, [vp, P(llp)]),

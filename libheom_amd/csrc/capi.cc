@@ -11,15 +11,16 @@
 #include "mlp.h"
 #include "solver.h"
 
+// The QME is shared with the solvers created on it: a solver keeps it alive
+// after heom_destroy, and checks its size before every solve.
 struct heom_qme {
-  std::unique_ptr<hamd::Qme> q;
+  std::shared_ptr<hamd::Qme> q;
 };
 struct heom_mlp {
   std::unique_ptr<hamd::Mlp> m;
 };
 struct heom_solver {
   std::unique_ptr<hamd::Solver> s;
-  heom_qme* owner;
 };
 
 namespace hamd {
@@ -125,7 +126,7 @@ int heom_create(int kind, int dtype, int n_level, int max_depth, const heom_opti
       throw hamd::HeomError{HEOM_ERR_UNSUPPORTED,
                             "col_major is provided for heom_hilb and the Redfield forms"};
     auto h = std::make_unique<heom_qme>();
-    h->q = std::make_unique<hamd::Qme>(kind, dtype, n_level, max_depth, o);
+    h->q = std::make_shared<hamd::Qme>(kind, dtype, n_level, max_depth, o);
     *out = h.release();
     return HEOM_OK;
   });
@@ -309,7 +310,7 @@ int heom_batch_create(const heom_handle* members, int n_members, heom_handle* ou
     std::vector<hamd::Qme*> ms;
     for (int m = 0; m < n_members; ++m) ms.push_back(&Qready(members[m]));
     auto h = std::make_unique<heom_qme>();
-    h->q = std::make_unique<hamd::Qme>(HEOM_HILB, ms[0]->dtype, 1, 0, ms[0]->opt);
+    h->q = std::make_shared<hamd::Qme>(HEOM_HILB, ms[0]->dtype, 1, 0, ms[0]->opt);
     h->q->set_param_batch(ms);
     *out = h.release();
     return HEOM_OK;
@@ -475,10 +476,9 @@ int heom_solver_create(heom_handle h, int kind, heom_solver_handle* out) {
   return guard([&] {
     if (!out) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solver_create: out is NULL"};
     *out = nullptr;
-    hamd::Qme& q = Qready(h);
+    (void)Qready(h);
     auto s = std::make_unique<heom_solver>();
-    s->s = std::make_unique<hamd::Solver>(&q, kind);
-    s->owner = h;
+    s->s = std::make_unique<hamd::Solver>(h->q, kind);
     *out = s.release();
     return HEOM_OK;
   });
@@ -497,6 +497,7 @@ int heom_solve_dev(heom_solver_handle s, void* rho_dev, const double* t_list, in
     if (!s || !s->s) throw hamd::HeomError{HEOM_ERR_ARG, "null solver"};
     if (!rho_dev || !t_list) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solve_dev: NULL"};
     hamd::Solver& sv = *s->s;
+    sv.check_qme();
     const bool batch = sv.q->kind == hamd::kBatch;
     const int N2 = batch ? sv.q->batch_rs_size : sv.q->N2;
     hamd::DevBuf rs;
@@ -529,6 +530,7 @@ int heom_solve(heom_solver_handle s, double* rho_host, const double* t_list, int
     if (!s || !s->s) throw hamd::HeomError{HEOM_ERR_ARG, "null solver"};
     if (!rho_host || !t_list) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solve: NULL"};
     hamd::Solver& sv = *s->s;
+    sv.check_qme();
     const long long ms = sv.q->main_size();
     const int N2 = sv.q->N2;
     const size_t bytes = sizeof(hamd::z128) * ms;
@@ -630,6 +632,32 @@ int heom_memcpy_h2d(void* dst, const void* src, long long bytes) {
 int heom_memcpy_d2h(void* dst, const void* src, long long bytes) {
   return guard([&] {
     HAMD_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost));
+    return HEOM_OK;
+  });
+}
+
+int heom_stream_copy(long long bytes, int grid, int reps, double* gbs) {
+  return guard([&] {
+    if (!gbs || bytes < 16 || grid < 1 || reps < 1)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_stream_copy: bad argument"};
+    const long long n = bytes / 16;
+    hamd::DevBuf src, dst;
+    src.alloc(static_cast<size_t>(n) * 16);
+    dst.alloc(static_cast<size_t>(n) * 16);
+    HAMD_HIP(hipMemset(src.p, 0, static_cast<size_t>(n) * 16));
+    hipEvent_t e0, e1;
+    HAMD_HIP(hipEventCreate(&e0));
+    HAMD_HIP(hipEventCreate(&e1));
+    hamd::launch_copy_grid(dst.p, src.p, n, grid, nullptr);  // warm-up
+    HAMD_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) hamd::launch_copy_grid(dst.p, src.p, n, grid, nullptr);
+    HAMD_HIP(hipEventRecord(e1, nullptr));
+    HAMD_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * 16.0 * static_cast<double>(n) * reps / (static_cast<double>(ms) * 1e-3) / 1e9;
     return HEOM_OK;
   });
 }

@@ -85,6 +85,7 @@ struct Ctrl {
   int count;    // attempts of the current step (rkdp.h:94-102)
   int gave_up;
   long long attempts, accepted, rejected;
+  long long projected;  // accepted attempts whose rho_0 the projection has handled
 };
 
 enum SlotType { ST_C128 = 0, ST_C64 = 1 };

@@ -56,11 +56,13 @@ void launch_resident_dopri(const FormDev& d, int max_links, const ResidentDopriA
 void launch_rho0_trace(const void* y, int N, double* tr0, hipStream_t stream);
 // rho_0 <- (rho_0 + rho_0^+)/2 * tr0 / Tr rho_0 in the FP64 state of parity
 // ctrl->cur (y0 / y1; parity 0 without ctrl), and its FP32 copy f0 / f1 if given
-void launch_rho0_renorm(void* y0, void* y1, void* f0, void* f1, const Ctrl* ctrl, int N,
+void launch_rho0_renorm(void* y0, void* y1, void* f0, void* f1, Ctrl* ctrl, int N,
                         const double* tr0, hipStream_t stream);
 // dst (FP32 complex) <- src (FP64 complex), n elements
 void launch_narrow(void* dst, const void* src, long long n, hipStream_t stream);
 // dst <- src for n complex elements of the given slot type
 void launch_copy(void* dst, const void* src, long long n, int type, hipStream_t stream);
+// dst <- src for n complex128 elements on a grid-stride grid of `grid` x 256 threads
+void launch_copy_grid(void* dst, const void* src, long long n, int grid, hipStream_t stream);
 
 }  // namespace hamd

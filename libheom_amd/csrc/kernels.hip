@@ -178,6 +178,13 @@ __device__ void dopri_control(Ctrl* c, double err) {
   }
   if (c->t + dt > c->t_bound) dt = c->t_bound - c->t;
   c->dt = dt;
+  // a step that is not positive would never reach t_bound: give up instead
+  // (unreachable from a positive step, which the rule above keeps positive)
+  if (!(dt > 0.0)) {
+    c->t = c->t_bound;
+    c->done = 1;
+    c->gave_up = 1;
+  }
 }
 
 // Block reduction of the error-norm partial sums: one value per workgroup,
@@ -1426,9 +1433,12 @@ __global__ void __launch_bounds__(1024) rho0_trace_kernel(const z128* y, int N, 
 }
 
 __global__ void __launch_bounds__(1024) rho0_renorm_kernel(z128* y0, z128* y1, z64* f0, z64* f1,
-                                                          const Ctrl* ctrl, int N,
+                                                          Ctrl* ctrl, int N,
                                                           const double* tr0) {
   __shared__ double s_factor;
+  // with a control block: project once after each accepted attempt, never after
+  // a rejected one or a no-op attempt of a finished interval
+  if (ctrl && ctrl->accepted == ctrl->projected) return;
   const int cur = ctrl ? ctrl->cur : 0;
   z128* y = cur ? y1 : y0;
   z64* f = cur ? f1 : f0;
@@ -1460,6 +1470,7 @@ __global__ void __launch_bounds__(1024) rho0_renorm_kernel(z128* y0, z128* y1, z
       if (f) f[e] = z64{static_cast<float>(w.re), static_cast<float>(w.im)};
     }
   }
+  if (ctrl && threadIdx.x == 0) ctrl->projected = ctrl->accepted;
 }
 
 // ---------------------------------------------------------------------------
@@ -1691,11 +1702,14 @@ __global__ void __launch_bounds__(1024)
       double dt = s_c.dt;
       s_c.t = t0;
       s_c.t_bound = t1;
-      s_c.done = 0;
       s_c.count = 1;
       s_c.gave_up = 0;
-      if (t0 + dt > t1) dt = t1 - t0;
-      s_c.dt = dt;
+      // a zero-length interval takes no attempt and keeps the step size
+      s_c.done = (t1 > t0) ? 0 : 1;
+      if (!s_c.done) {
+        if (t0 + dt > t1) dt = t1 - t0;
+        s_c.dt = dt;
+      }
     }
     __syncthreads();
     while (!s_c.done) {
@@ -1771,6 +1785,9 @@ __global__ void __launch_bounds__(1024)
       }
       __syncthreads();
     }
+    // every wave has read done = 1 before thread 0 rewrites the control block
+    // for the next interval
+    __syncthreads();
     if (e < N2) a.rho_s[static_cast<size_t>(iv + 1) * N2 + e] = buf[(s_c.cur ? ms : 0) + e];
   }
   if (ok) a.y[e] = buf[(s_c.cur ? ms : 0) + e];
@@ -2068,7 +2085,7 @@ void launch_rho0_trace(const void* y, int N, double* tr0, hipStream_t stream) {
   HAMD_HIP(hipGetLastError());
 }
 
-void launch_rho0_renorm(void* y0, void* y1, void* f0, void* f1, const Ctrl* ctrl, int N,
+void launch_rho0_renorm(void* y0, void* y1, void* f0, void* f1, Ctrl* ctrl, int N,
                         const double* tr0, hipStream_t stream) {
   if (N > 64) throw HeomError{HEOM_ERR_UNSUPPORTED, "renorm_rho0: n_level > 64"};
   hipLaunchKernelGGL(rho0_renorm_kernel, dim3(1), dim3(1024), 0, stream, static_cast<z128*>(y0),
@@ -2159,6 +2176,12 @@ void launch_resident_dopri(const FormDev& d, int max_links, const ResidentDopriA
 void launch_narrow(void* dst, const void* src, long long n, hipStream_t stream) {
   hipLaunchKernelGGL(narrow_kernel, dim3(cap_grid((n + 255) / 256)), dim3(256), 0, stream,
                      static_cast<z64*>(dst), static_cast<const z128*>(src), n);
+  HAMD_HIP(hipGetLastError());
+}
+
+void launch_copy_grid(void* dst, const void* src, long long n, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL((copy_kernel<z128>), dim3(grid), dim3(256), 0, stream, static_cast<z128*>(dst),
+                     static_cast<const z128*>(src), n);
   HAMD_HIP(hipGetLastError());
 }
 

@@ -198,7 +198,7 @@ void Qme::set_param() {
   if (static_cast<long long>(nh) * N2 > (1LL << 31) - 1)
     throw HeomError{HEOM_ERR_UNSUPPORTED, "set_param: main_size exceeds 2^31"};
 
-  // Gamma_n (heom.h:177-188)
+  // Gamma_n (heom.h:93-104)
   ngamma.assign(nh, z128{0.0, 0.0});
   for (int l = 0; l < nh; ++l) {
     z128 g{0.0, 0.0};
@@ -209,7 +209,7 @@ void Qme::set_param() {
     ngamma[l] = g;
   }
 
-  // s = S phi_0, a = A phi_0 (heom.h:207-222)
+  // s = S phi_0, a = A phi_0 (heom.h:123-138)
   s.assign(nn, {});
   a.assign(nn, {});
   for (int u = 0; u < nn; ++u) {

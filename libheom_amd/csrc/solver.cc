@@ -48,7 +48,9 @@ void StageSpec::combo(const std::vector<std::pair<int, double>>& terms, double c
   dst2[c] = d2;
 }
 
-Solver::Solver(Qme* q_, int kind_) : q(q_), kind(kind_) {
+Solver::Solver(std::shared_ptr<Qme> q_, int kind_)
+    : q_owner(std::move(q_)), q(q_owner.get()), kind(kind_) {
+  if (!q) throw HeomError{HEOM_ERR_ARG, "solver: null QME"};
   if (!q->ready) throw HeomError{HEOM_ERR_STATE, "solver: set_param has not been called"};
   if (kind != HEOM_RK4 && kind != HEOM_RKDP && kind != HEOM_LSRK4)
     throw HeomError{HEOM_ERR_ARG, "solver: unknown solver kind"};
@@ -96,6 +98,15 @@ Solver::Solver(Qme* q_, int kind_) : q(q_), kind(kind_) {
 
 Solver::~Solver() {
   if (h_ctrl) (void)hipHostFree(h_ctrl);
+}
+
+void Solver::check_qme() const {
+  if (!q->ready)
+    throw HeomError{HEOM_ERR_STATE, "solver: the QME's parameters changed and set_param has not "
+                                    "been called again"};
+  if (q->main_size() != ms)
+    throw HeomError{HEOM_ERR_STATE, "solver: the QME's main_size changed since the solver was "
+                                    "created; create a new solver"};
 }
 
 // Physical buffer of a slot for ping-pong parity `cur` (slots 0..3 swap
@@ -432,8 +443,8 @@ void Solver::rk4_interval(double t0, double t1, double dt) {
 int Solver::rkdp_interval(double t0, double t1, double dt_first, int& batch) {
   launch_ctrl_init(d_ctrl.as<Ctrl>(), t0, t1, dt_first, q->stream);
   long long a0 = h_ctrl->attempts;
-  // a rejected attempt leaves the projected state unchanged, so projecting after
-  // every attempt equals projecting after every accepted step (up to rounding)
+  // the projection kernel runs after every attempt and acts only when the
+  // control block counts a new accepted step
   const auto attempt = [&] {
     for (int s = 0; s < 7; ++s) launch_stage(s, 0.0);
     if (q->opt.renorm_rho0) renorm(true);
@@ -447,7 +458,7 @@ int Solver::rkdp_interval(double t0, double t1, double dt_first, int& batch) {
     HAMD_HIP(hipMemcpyAsync(h_ctrl, d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, q->stream));
     HAMD_HIP(hipStreamSynchronize(q->stream));
     if (h_ctrl->done) break;
-    batch = std::max(1, batch * 2);
+    batch = std::min(kMaxAttemptBatch, std::max(1, batch * 2));
   }
   long long used = h_ctrl->attempts - a0;
   batch = static_cast<int>(std::max<long long>(1, std::min<long long>(used + 1, 4096)));
@@ -456,6 +467,7 @@ int Solver::rkdp_interval(double t0, double t1, double dt_first, int& batch) {
 
 int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, double atol,
                   double rtol, const std::function<void(int, const void*)>& on_output) {
+  check_qme();
   if (n_t < 1) throw HeomError{HEOM_ERR_ARG, "solve: n_t < 1"};
   for (int i = 0; i + 1 < n_t; ++i)
     if (!(t_list[i + 1] >= t_list[i]))
@@ -492,13 +504,18 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
     double first = dt_save > 0.0 ? dt_save : dt;
     int batch = 4;
     int cur = 0;
+    bool started = false;
     for (int i = 0; i < n_t - 1; ++i) {
       on_output(i, P[cur].p);
-      gave_up |= rkdp_interval(t_list[i], t_list[i + 1], i == 0 ? first : -1.0, batch);
+      // a zero-length interval takes no attempt and keeps the step size (the
+      // reference would clamp the step to 0 and never finish the next interval)
+      if (!(t_list[i + 1] > t_list[i])) continue;
+      gave_up |= rkdp_interval(t_list[i], t_list[i + 1], started ? -1.0 : first, batch);
+      started = true;
       cur = h_ctrl->cur;
     }
     on_output(n_t - 1, P[cur].p);
-    dt_save = h_ctrl->dt;
+    if (started) dt_save = h_ctrl->dt;
     stats.n_rhs += 7 * h_ctrl->attempts;
     stats.n_attempts += h_ctrl->attempts;
     stats.n_accepted += h_ctrl->accepted;
@@ -515,6 +532,7 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
 
 int Solver::solve_gather(void* rho_dev, const double* t_list, int n_t, double dt, double atol,
                          double rtol, void* rho_s_dev) {
+  check_qme();
   const size_t rs_bytes = sizeof(z128) * q->N2;
   hipStream_t st = q->stream;
   if (kind == HEOM_RKDP && n_t >= 2 && use_graphs && in_bytes == 8 && !q->opt.renorm_rho0 &&
@@ -573,6 +591,7 @@ int Solver::solve_gather(void* rho_dev, const double* t_list, int n_t, double dt
 // small problem: every interval, attempt and output in one resident launch.
 int Solver::solve_resident_dopri(void* rho_dev, const double* t_list, int n_t, double dt,
                                  double atol, double rtol, void* rho_s_dev) {
+  check_qme();
   for (int i = 0; i + 1 < n_t; ++i)
     if (!(t_list[i + 1] >= t_list[i]))
       throw HeomError{HEOM_ERR_ARG, "solve: t_list must be non-decreasing"};
@@ -601,7 +620,12 @@ int Solver::solve_resident_dopri(void* rho_dev, const double* t_list, int n_t, d
   HAMD_HIP(hipMemcpyAsync(rho_dev, P[SL_Y].p, bytes, hipMemcpyDeviceToDevice, st));
   HAMD_HIP(hipMemcpyAsync(h_ctrl, d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
   HAMD_HIP(hipStreamSynchronize(st));
-  dt_save = h_ctrl->dt;
+  // only intervals of positive length touch the step size (as in solve)
+  for (int i = 0; i + 1 < n_t; ++i)
+    if (t_list[i + 1] > t_list[i]) {
+      dt_save = h_ctrl->dt;
+      break;
+    }
   stats.n_rhs += 7 * h_ctrl->attempts;
   stats.n_attempts += h_ctrl->attempts;
   stats.n_accepted += h_ctrl->accepted;
@@ -614,6 +638,7 @@ int Solver::solve_resident_dopri(void* rho_dev, const double* t_list, int n_t, d
 }
 
 void Solver::time_stages(int reps, std::vector<double>& out) {
+  check_qme();
   HAMD_HIP(hipSetDevice(q->opt.device));
   hipStream_t st = q->stream;
   const int ns = n_stages();

@@ -45,9 +45,10 @@ struct GraphExec {
 
 class Solver {
  public:
-  Solver(Qme* q, int kind);
+  Solver(std::shared_ptr<Qme> q, int kind);
   ~Solver();
 
+  std::shared_ptr<Qme> q_owner;  // keeps the QME alive while the solver exists
   Qme* q;
   int kind;        // HEOM_RK4 / HEOM_RKDP / HEOM_LSRK4
   int in_bytes;    // real size of the operator arithmetic and stage inputs (8 or 4)
@@ -70,6 +71,9 @@ class Solver {
 
   // Average duration (ms) of each stage kernel over `reps` replays.
   void time_stages(int reps, std::vector<double>& ms_out);
+  // throws HEOM_ERR_STATE unless the QME is set up with the size the solver's
+  // buffers were allocated for (set_param may have been called again)
+  void check_qme() const;
   void stage_bytes(std::vector<double>& bytes_out);
   int n_stages() const { return kind == HEOM_RKDP ? 7 : 4; }
 
@@ -94,6 +98,8 @@ class Solver {
   // graphs of 2^j full RK4 steps / Dormand-Prince attempts (j <= 8), by 2^j;
   // a count is replayed as its binary decomposition
   static constexpr int kMaxGraphSteps = 256;
+  // most Dormand-Prince attempts enqueued before the host reads the control block
+  static constexpr int kMaxAttemptBatch = 1 << 14;
   static constexpr size_t kMaxGraphs = 9;
   std::map<long long, std::unique_ptr<GraphExec>> g_rk4, g_dopri;
   double g_rk4_dt = 0.0;

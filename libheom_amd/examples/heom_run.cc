@@ -12,10 +12,17 @@
 //   solver(0 rk4|1 rkdp|2 lsrk4) dt atol rtol n_t t_list[n_t]
 //   rho0[main_size]
 // Output: one line per output time: t followed by rho_S (N*N complex, re im).
+//
+// heom_run model.txt calc_diff: instead of solving, print one line holding
+//   qme_solver::calc_diff(drho, rho0), i.e. drho = -D(rho0) (main_size complex).
+// heom_run model.txt reparam: solve, then halve H, call set_param again on the
+//   same qme object and solve again with the same qme_solver (both runs'
+//   output lines are printed, the second after a line "reparam").
 #include <cstdio>
 #include <fstream>
 #include <iostream>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "libheom_amd.hpp"
@@ -41,9 +48,10 @@ static void fill(std::istream& in, lil_matrix<dtype>& m, int rows, int cols) {
 
 int main(int argc, char** argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: heom_run model.txt\n");
+    std::fprintf(stderr, "usage: heom_run model.txt [calc_diff|reparam]\n");
     return 2;
   }
+  const std::string mode = argc > 2 ? argv[2] : "";
   std::ifstream in(argv[1]);
   int form, N, depth, n_noise;
   in >> form >> N >> depth >> n_noise;
@@ -95,15 +103,30 @@ int main(int argc, char** argv) {
     else
       solver.reset(new rkdp<dtype, row_major, engine>());
     qme_solver<dtype, row_major, engine> qs(&eng, qme.get(), solver.get());
+    if (mode == "calc_diff") {
+      std::vector<dtype> drho(rho.size());
+      qs.calc_diff(drho.data(), rho.data());
+      for (size_t k = 0; k < drho.size(); ++k)
+        std::printf("%s%.17g %.17g", k ? " " : "", drho[k].real(), drho[k].imag());
+      std::printf("\n");
+      return 0;
+    }
+    const std::vector<dtype> rho0 = rho;
     kwargs_t kw{{"dt", dt}, {"atol", atol}, {"rtol", rtol}};
-    qs.solve(rho.data(), t_list.data(), n_t,
-             [&](double t) {
-               std::printf("%.17g", t);
-               for (int k = 0; k < N * N; ++k)
-                 std::printf(" %.17g %.17g", rho[k].real(), rho[k].imag());
-               std::printf("\n");
-             },
-             kw);
+    auto print = [&](double t) {
+      std::printf("%.17g", t);
+      for (int k = 0; k < N * N; ++k) std::printf(" %.17g %.17g", rho[k].real(), rho[k].imag());
+      std::printf("\n");
+    };
+    qs.solve(rho.data(), t_list.data(), n_t, print, kw);
+    if (mode == "reparam") {
+      for (auto& r : qme->H.data)
+        for (auto& c : r.second) c.second *= 0.5;
+      qme->set_param(&eng);
+      rho = rho0;
+      std::printf("reparam\n");
+      qs.solve(rho.data(), t_list.data(), n_t, print, kw);
+    }
   } catch (heom_error& e) {
     std::fprintf(stderr, "heom_error %d: %s\n", e.code, e.what());
     return 1;

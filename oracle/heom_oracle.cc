@@ -522,7 +522,7 @@ void set_param(Model& m) {
 
   alloc_hrchy(m);
 
-  // ngamma_diag (heom.h:177-188)
+  // ngamma_diag (heom.h:93-104)
   m.ngamma_diag.assign(m.n_hrchy, 0.0);
   for (int l = 0; l < m.n_hrchy; ++l) {
     cplx g = 0.0;
@@ -533,7 +533,7 @@ void set_param(Model& m) {
     m.ngamma_diag[l] = g;
   }
 
-  // gamma_offdiag (heom.h:190-205).  In the reference, set_shape clears
+  // gamma_offdiag (heom.h:106-121; set_shape at 110).  In the reference, set_shape clears
   // gamma[u] before the copy loop, so gamma_offdiag is always empty; the
   // parity flag reproduces that, the non-parity mode keeps the intended term.
   m.gamma_offdiag.assign(nn, {});
@@ -548,7 +548,7 @@ void set_param(Model& m) {
     }
   }
 
-  // s = S phi_0, a = A phi_0 (heom.h:207-222; nil gemv, linalg_engine_nil.h.j2:137-164)
+  // s = S phi_0, a = A phi_0 (heom.h:123-138; nil gemv, linalg_engine_nil.h.j2:137-164)
   m.s.assign(nn, {});
   m.a.assign(nn, {});
   for (int u = 0; u < nn; ++u) {
@@ -1187,6 +1187,11 @@ int orc_solve(void* p, int solver, double* rho, const double* t_list, int n_t, d
     if (solver != 1) {
       fixed_solve_1(*m, r, t_list[i], t_list[i + 1], dt, buf, st, solver == 2);
     } else {
+      // Zero-length interval: no attempt, dt_save kept.  The reference would
+      // take one attempt of dt = 0 (state unchanged), save dt_save = 0 and then
+      // never finish the next interval; the library skips such intervals and so
+      // does the oracle.
+      if (!(t_list[i + 1] > t_list[i])) continue;
       if (dt_save == -1.0) dt_save = dt;
       double t = t_list[i];
       double dt_1 = dt_save;

@@ -55,3 +55,56 @@ def test_heom_run_matches_oracle(tmp_path, form, solver):
     _, rs_o, _, _, _ = o.solve(m["rho0"], t_list, [O.RK4, O.RKDP, O.LSRK4][solver], dt, atol,
                                rtol)
     assert np.abs(rs - rs_o).max() <= 1e-10
+
+
+def _rows(text):
+    return np.array([[float(x) for x in line.split()] for line in text.strip().split("\n")])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_heom_run_calc_diff_matches_oracle(tmp_path, form):
+    """qme_solver::calc_diff (the reference's stub, qme_solver.h:37-42) computes
+    drho = -D(rho) through the C++ drop-in."""
+    m = configs.random_model(3, 2, 2, 3, seed=4, dense_v=True)
+    rng = np.random.default_rng(8)
+    rho = (rng.normal(size=m["rho0"].size) + 1j * rng.normal(size=m["rho0"].size)) * 0.1
+    f = tmp_path / "model.txt"
+    write_model(str(f), form, m, 0, 0.01, 0.0, 0.0, [0.0, 0.1], rho)
+    print("heom_run calc_diff", form, flush=True)
+    r = subprocess.run([EXE, str(f), "calc_diff"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    row = _rows(r.stdout)[0]
+    got = row[0::2] + 1j * row[1::2]
+    ref = O.Oracle(form, 3, m["max_depth"]).load(m).calc_diff(rho, 1.0, 0.0)
+    assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", [0, 1])
+def test_heom_run_set_param_again_on_the_same_qme(tmp_path, solver):
+    """set_param called again on the same qme (H halved) while a qme_solver
+    holds it: the second solve uses the new parameters, as the reference's
+    qme_solver (a pointer to the same qme) does."""
+    m = configs.spin_boson(max_depth=5)
+    t_list = np.linspace(0.0, 1.0, 6)
+    dt, atol, rtol = (0.05, 1e-10, 1e-8) if solver == 1 else (0.01, 0.0, 0.0)
+    f = tmp_path / "model.txt"
+    write_model(str(f), 0, m, solver, dt, atol, rtol, t_list, m["rho0"])
+    print("heom_run reparam", solver, flush=True)
+    r = subprocess.run([EXE, str(f), "reparam"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    first, second = r.stdout.split("reparam\n")
+    okind = [O.RK4, O.RKDP][solver]
+    for text, scale in ((first, 1.0), (second, 0.5)):
+        rows = _rows(text)
+        rs = (rows[:, 1::2] + 1j * rows[:, 2::2]).reshape(len(t_list), 2, 2)
+        m2 = dict(m, H=m["H"] * scale)
+        o = O.Oracle(O.HILB, 2, m["max_depth"]).load(m2)
+        # the adaptive solver keeps dt_save across the two solves
+        dts = None
+        if solver == 1 and scale != 1.0:
+            _, _, _, dts, _ = O.Oracle(O.HILB, 2, m["max_depth"]).load(m).solve(
+                m["rho0"], t_list, okind, dt, atol, rtol)
+        _, rs_o, _, _, _ = o.solve(m["rho0"], t_list, okind, dt, atol, rtol, dt_save=dts)
+        assert np.abs(rs - rs_o).max() <= 1e-10, scale

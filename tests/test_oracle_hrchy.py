@@ -1,11 +1,13 @@
 """Pins the oracle's hierarchy (alloc_hrchy_space, hrchy_space.cc:137-231).
 
-Checks: the reference's own src/hrchy_space.cc compiled into oracle/_ref
-(identical breadth-first order and +-1 tables), the closed-form count
-C(K+L, L), and the table invariants (uniqueness, tiers, ptr_m1[ptr_p1] = id,
-sentinel row).
+Checks: the reference's own src/hrchy_space.cc -- live when it is compiled into
+oracle/_ref, and always through tests/golden/hrchy_ref.npz, the tables it
+produced (identical breadth-first order and +-1 tables, with and without an
+hrchy_filter) -- the closed-form count C(K+L, L), and the table invariants
+(uniqueness, tiers, ptr_m1[ptr_p1] = id, sentinel row).
 """
 import itertools
+import os
 from math import comb
 
 import numpy as np
@@ -13,8 +15,9 @@ import pytest
 
 from libheom_amd import configs
 from oracle import oracle as O
+from tests.golden.make_hrchy_golden import CASES, FILTER_CASES, FILTERS, key
 
-CASES = [(1, 0), (1, 5), (2, 10), (3, 4), (4, 3), (5, 5), (8, 3), (21, 2), (16, 3), (7, 4)]
+GOLDEN = np.load(os.path.join(os.path.dirname(__file__), "golden", "hrchy_ref.npz"))
 
 
 def oracle_tables(K, L):
@@ -36,6 +39,12 @@ def test_matches_reference_hrchy_space(K, L):
         pytest.skip("oracle/_ref not built (reference sources absent)")
     for a, b in zip(oracle_tables(K, L), ref):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("K,L", CASES)
+def test_matches_reference_fixture(K, L):
+    for tag, a in zip(("n", "p1", "m1"), oracle_tables(K, L)):
+        assert np.array_equal(a, GOLDEN[key(K, L) + "_" + tag]), tag
 
 
 @pytest.mark.parametrize("K,L", CASES)
@@ -79,17 +88,19 @@ def test_every_multi_index_present():
 
 
 # hrchy_filter (hrchy_space.h:29-36): pinned against the reference's own
-# alloc_hrchy_space compiled from src/hrchy_space.cc
-FILTERS = {
-    "max occupation 2": lambda n, d: n.max() <= 2,
-    "weighted depth": lambda n, d: int((n * (np.arange(n.size) + 1)).sum()) <= 7,
-    # rejects the intermediate index (1, 0, ...): its canonical descendants vanish too
-    "prune through an intermediate": lambda n, d: not (n[0] == 1 and n[1:].sum() == 0),
-}
+# alloc_hrchy_space compiled from src/hrchy_space.cc (live and through the fixture)
+@pytest.mark.parametrize("name", sorted(FILTERS))
+@pytest.mark.parametrize("K,L", FILTER_CASES)
+def test_filtered_hierarchy_matches_reference_fixture(name, K, L):
+    fn = FILTERS[name]
+    m = configs.random_model(2, 1, K, L, seed=K, dense_v=False)
+    o = O.Oracle(O.HILB, 2, L).set_filter(fn).load(m)
+    for tag, a in zip(("n", "p1", "m1"), o.tables()):
+        assert np.array_equal(a, GOLDEN[key(K, L, name) + "_" + tag]), tag
 
 
 @pytest.mark.parametrize("name", sorted(FILTERS))
-@pytest.mark.parametrize("K,L", [(3, 5), (4, 4), (6, 3)])
+@pytest.mark.parametrize("K,L", FILTER_CASES)
 def test_filtered_hierarchy_matches_reference(name, K, L):
     fn = FILTERS[name]
     ref = O.ref_hrchy_filtered(K, L, fn)

@@ -1,0 +1,123 @@
+"""Full-size workloads against the oracle's committed outputs (tests/golden/).
+
+The oracle (an FP64 restatement of heom_hilb.h:70-215, rkdp.h:81-179,
+rk4.h:40-65, solver_base.h:64-91) needs minutes at these sizes, so its
+results were written once by tests/golden/make_golden.py:
+
+- the 7-site chain exactly as benchmarked (12650 ADOs, Dormand-Prince rtol 1e-8,
+  atol 1e-11, 0..1000 fs, rho_S every 10 fs): the plain FP64 form must match
+  rho_S at all 101 output times to 1e-10 max-abs with the oracle's accepted and
+  rejected attempt counts, through graph replay and through direct launches;
+  the complex64 form states its error (bar 1e-6);
+- the N=64 config (969 ADOs of 64 x 64, Dormand-Prince rtol 1e-6) over t = 0..2:
+  FP64 to 1e-10 with identical counts; complex64 and split-bf16x3 state their
+  errors (bar 1e-6);
+- 24 members of the batched sweep through all 4000 RK4 steps in one batch:
+  FP64 rho_S to 1e-10 relative to the member's magnitude (members that leave
+  RK4's stability region must diverge on the GPU too); the complex64 batch
+  states its error on the members whose whole hierarchy stays below 1e4.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import libheom_amd as H
+from libheom_amd import configs
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden(name):
+    z = np.load(os.path.join(HERE, name))
+    return z, json.loads(str(z["meta"]))
+
+
+def run(m, dtype, graphs, t_list=None):
+    q = H.heom_hilb(m["max_depth"], dtype=dtype).load(m, H.hip(0))
+    s = H.qme_solver(H.hip(0), q, H.rkdp())
+    s.set_graphs(graphs)
+    t = m["t_list"] if t_list is None else t_list
+    rho, rs = s.solve_rho_s(m["rho0"], t, {"dt": m["dt"], "atol": m["atol"], "rtol": m["rtol"]})
+    return rho, rs, s.stats()
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_chain7_full_dopri_fp64(graphs):
+    z, meta = golden("chain7_dopri.npz")
+    m = configs.exciton_chain7()
+    assert np.array_equal(m["t_list"], z["t_list"])
+    print("chain7 full Dormand-Prince, graphs=%s" % graphs, flush=True)
+    rho, rs, st = run(m, "complex128", graphs)
+    err = float(np.abs(rs - z["rho_s"]).max())
+    print("rho_S max-abs vs oracle %.3e; accepted %d rejected %d (oracle %d %d)"
+          % (err, st["n_accepted"], st["n_rejected"], meta["accepted"], meta["rejected"]))
+    assert (st["n_accepted"], st["n_rejected"]) == (meta["accepted"], meta["rejected"])
+    assert st["n_rhs"] == meta["n_rhs"]
+    assert err <= TOL
+    assert abs(st["last_dt"] - meta["dt_save"]) <= 1e-9 * meta["dt_save"]
+
+
+def test_chain7_full_dopri_complex64_error():
+    z, _ = golden("chain7_dopri.npz")
+    m = configs.exciton_chain7()
+    print("chain7 full Dormand-Prince, complex64", flush=True)
+    _, rs, _ = run(m, "complex64", True)
+    err = float(np.abs(rs - z["rho_s"]).max())
+    print("complex64 rho_S max-abs vs FP64 oracle %.3e" % err)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("dtype,bar", [("complex128", TOL), ("complex64", 1e-6),
+                                       ("complex64_bf16x3", 1e-6)])
+def test_dense64_dopri(dtype, bar):
+    z, meta = golden("dense64_dopri.npz")
+    m = configs.dense64(t_final=2.0)
+    assert np.array_equal(m["t_list"], z["t_list"])
+    print("dense64 Dormand-Prince t=0..2", dtype, flush=True)
+    _, rs, st = run(m, dtype, True)
+    err = float(np.abs(rs - z["rho_s"]).max())
+    print("%s rho_S max-abs vs FP64 oracle %.3e; accepted %d rejected %d (oracle %d %d)"
+          % (dtype, err, st["n_accepted"], st["n_rejected"], meta["accepted"], meta["rejected"]))
+    assert err <= bar
+    if dtype == "complex128":
+        assert (st["n_accepted"], st["n_rejected"]) == (meta["accepted"], meta["rejected"])
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_sweep24_rk4_4000_steps(dtype):
+    z, meta = golden("sweep24_rk4.npz")
+    ms = configs.batched_sweep(n_inst=24, seed=24, n_steps=4000)
+    assert [m["name"] for m in ms] == meta["names"]
+    b = H.heom_batch([H.heom_hilb(m["max_depth"], dtype=dtype).load(m, H.hip(0)) for m in ms])
+    s = H.qme_solver(H.hip(0), b, H.rk4())
+    t = ms[0]["t_list"]
+    print("sweep24 RK4 x 4000", dtype, flush=True)
+    rho, rs = s.solve_rho_s(b.pack([m["rho0"] for m in ms]), t, {"dt": ms[0]["dt"]})
+    assert s.stats()["n_rhs"] == meta["n_rhs"][0]
+    ref_all = z["rho_s"]
+    offs = np.concatenate([[0], np.cumsum(z["n2"])])
+    worst_stable, n_div, n_stable = 0.0, 0, 0
+    for k, (m, got_rho, got_rs) in enumerate(zip(ms, b.unpack(rho), b.split_rho_s(rs))):
+        ref = ref_all[:, offs[k]:offs[k + 1]].reshape(got_rs.shape)
+        scale = float(z["final_max_abs"][k])
+        if not np.isfinite(scale) or scale > 1e30:
+            n_div += 1
+            with np.errstate(all="ignore"):
+                assert not np.isfinite(got_rho).all() or np.abs(got_rho).max() > 1e25, m["name"]
+            continue
+        err = float(np.abs(got_rs - ref).max())
+        if scale <= 1e4:
+            n_stable += 1
+            worst_stable = max(worst_stable, err)
+        if dtype == "complex128":
+            bar = TOL * max(1.0, float(np.abs(ref).max())) if scale <= 1e4 else TOL * scale
+            assert err <= bar, (m["name"], err, scale)
+    print("%s: %d stable members, worst rho_S error %.3e; %d members outside RK4 stability"
+          % (dtype, n_stable, worst_stable, n_div))
+    assert n_stable >= 8
+    if dtype == "complex64":
+        assert worst_stable <= 1e-6

@@ -164,6 +164,7 @@ class qme_base {
       check(heom_create(kind_, dt, n_level, max_depth_, &opt, &h_));
       h_opt_ = opt;
       h_n_level_ = n_level;
+      h_gen_ = next_generation();
     }
     check(heom_set_hrchy_filter(h_, nullptr, nullptr));
     check(heom_set_H(h_, H.dense(n_level, n_level).data()));
@@ -196,6 +197,8 @@ class qme_base {
   }
 
   heom_handle handle() const { return h_; }
+  // changes whenever set_param replaces the handle (addresses may be reused)
+  unsigned long long handle_generation() const { return h_gen_; }
 
   // hrchy_filter of alloc_hrchy_space (hrchy_space.h:29-36); empty: none
   std::function<bool(std::vector<int>, int)> hrchy_filter;
@@ -217,6 +220,11 @@ class qme_base {
   heom_handle h_ = nullptr;
   heom_options h_opt_{};  // create-time options of h_
   int h_n_level_ = 0;
+  unsigned long long h_gen_ = 0;
+  static unsigned long long next_generation() {
+    static unsigned long long g = 0;
+    return ++g;
+  }
 };
 
 template <typename dtype, order_t order, typename engine>
@@ -335,7 +343,7 @@ class qme_solver {
              solver_base<dtype, order, engine>* solver_)
       : eng(engine_), qme(qme_), solver(solver_) {
     check(heom_solver_create(qme->handle(), solver->kind(), &s_));
-    qh_ = qme->handle();
+    qgen_ = qme->handle_generation();
   }
   ~qme_solver() {
     if (s_) heom_solver_destroy(s_);
@@ -392,12 +400,12 @@ class qme_solver {
       (*c->sync)();
       if (*c->cb) (*c->cb)(static_cast<real_t<dtype>>(t));
     };
-    if (qme->handle() != qh_) {  // set_param replaced the qme's handle: rebind
+    if (qme->handle_generation() != qgen_) {  // set_param replaced the qme's handle: rebind
       heom_solver_handle s2 = nullptr;
       check(heom_solver_create(qme->handle(), solver->kind(), &s2));
       heom_solver_destroy(s_);
       s_ = s2;
-      qh_ = qme->handle();
+      qgen_ = qme->handle_generation();
     }
     int rc = heom_solve(s_, buf.data(), ts.data(), n_t, trampoline, &ctx, dt, atol, rtol, 1);
     from_buf();
@@ -412,7 +420,7 @@ class qme_solver {
 
  private:
   heom_solver_handle s_ = nullptr;
-  heom_handle qh_ = nullptr;  // the qme handle s_ was created on
+  unsigned long long qgen_ = 0;  // generation of the qme handle s_ was created on
 };
 
 }  // namespace libheom

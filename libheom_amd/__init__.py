@@ -57,6 +57,9 @@ class hip:
 
 class _heom:
     _kind = None
+    # every new C handle gets a fresh generation number, so that a solver can
+    # tell a replaced handle from its own even when the allocator reuses the address
+    _generations = iter(range(1, 1 << 62))
 
     def __init__(self, max_depth, n_inner_threads=1, n_outer_threads=1, dtype="complex128",
                  order=row_major, parity_q1=True, sqrt_normalization=False, matrix_cores=True,
@@ -132,6 +135,7 @@ class _heom:
                                       ctypes.byref(h)))
             self._h = h
             self._hkey = key
+            self._hgen = next(_heom._generations)
         # the same handle is set up again in place, so solvers created on this
         # qme see the new parameters (qme_solver holds the qme, qme_solver.h:18-53)
         h = self._h
@@ -400,7 +404,7 @@ class qme_solver:
         if getattr(self, "_s", None):
             _capi.load().heom_solver_destroy(self._s)
         self._s = s
-        self._qh = self.qme._h.value
+        self._qgen = getattr(self.qme, "_hgen", 0)
         if not self._graphs:
             _capi.check(_capi.load().heom_solver_set_graphs(self._s, 0))
 
@@ -410,7 +414,7 @@ class qme_solver:
         whose buffers must still fit its main_size (else HEOM_ERR_STATE)."""
         if not self.qme._ready:
             raise HeomError(_capi.HEOM_ERR_STATE, "set_param has not been called")
-        if self.qme._h.value != self._qh:
+        if getattr(self.qme, "_hgen", 0) != self._qgen:
             self._bind()
 
     def __del__(self):

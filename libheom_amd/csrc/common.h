@@ -142,6 +142,14 @@ struct alignas(16) LinkFlat {
   z128 iphi, psi, dir;
 };
 
+// Work unit of the one-ADO-per-workgroup operator kernels: workgroup b
+// computes ADO g, whose links are links[l0 .. l1).  Workgroups are dealt to the
+// eight XCDs round-robin (b mod 8), so the map places ADOs that share
+// neighbours on one XCD (one L2) -- see Qme::build_block_map.
+struct alignas(16) BlockMap {
+  int g, l0, l1, pad;
+};
+
 // One member of a batch of independent Hilbert-form hierarchies packed into
 // one set of device tables (Qme::set_param_batch).
 struct BatchInst {
@@ -181,6 +189,7 @@ struct FormDev {
   int real_v;            // diagonal V with real entries
   int max_links;         // most links of any single ADO
   const LinkFlat* flat;  // links with inlined coefficients (same order as links)
+  const BlockMap* bmap;  // n_hrchy work units, one per workgroup (single hierarchies)
   // diagonal V: per noise u (plus one zero row) and element e = i*N+j the pair
   // (v_i - v_j, v_i + v_j); real V stores the pair as (re, im) of one z128,
   // complex V as two z128

@@ -132,10 +132,17 @@ __device__ __forceinline__ double epilogue(const PreT<KP>& p, const StageArgs& A
       if (void* d2 = pick(A.dst2[c], cur)) stz(d2, F64 ? false : A.dst2_lo[c], gi, v);
     }
   }
-  if (KP == kMaxPre && A.do_err) {
-    // operands KP-2 / KP-1 hold y_new / y_old (zero weights in ec)
-    const z128 e = combine(p, A.ec, KP - 2, A.ek, k);
-    const double a1 = zabs(p.v[KP - 2]), a0 = zabs(p.v[KP - 1]);
+  if (A.do_err) {
+    // the last two operands hold y_new / y_old (zero weights in ec)
+    const int np = A.npre;
+    const z128 e = combine(p, A.ec, np - 2, A.ek, k);
+    z128 yn{0.0, 0.0}, yo{0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KP; ++s) {
+      if (s == np - 2) yn = p.v[s];
+      if (s == np - 1) yo = p.v[s];
+    }
+    const double a1 = zabs(yn), a0 = zabs(yo);
     const double w = zabs(e) / (A.atol + A.rtol * (a1 > a0 ? a1 : a0));
     return w * w;
   }
@@ -271,6 +278,13 @@ __device__ __forceinline__ void add_closure(const FormDev& d, long long gi, cx<R
 // ---------------------------------------------------------------------------
 // heom_hilb, fast path: one ADO per workgroup (N = 6..16), diagonal V
 // (heom_hilb.h:96-212 with [V,X]_ij = (v_i - v_j) X_ij, {V,X}_ij = (v_i + v_j) X_ij)
+//
+// Workgroup b takes the ADO of work unit bmap[b] (its link range in the same
+// record), read beside the control block.  H and the diagonals of every V_u
+// are staged in LDS once per ADO (no per-link coefficient tables), each link is
+// one 64-byte scalar record load and one neighbour-element load.
+// REAL: H and every V_u real (the commutator and the link coefficients in real
+// arithmetic on the real parts; a complex H or V takes REAL = false).
 // ---------------------------------------------------------------------------
 template <int N>
 struct WaveCfg {
@@ -278,14 +292,26 @@ struct WaveCfg {
   static constexpr int BS = ((N2 + 63) / 64) * 64;
 };
 
-template <int N, typename R, bool REALV, bool F64, int KP>
-__global__ void __launch_bounds__(WaveCfg<N>::BS)
+// LDS bytes of the diagonals of every V_u (plus one zero row for direct links)
+inline size_t fast_vd_lds(const FormDev& d, bool real, int real_bytes) {
+  const size_t rows = static_cast<size_t>(d.n_noise + 1) * d.N;
+  return rows * (real ? 1 : 2) * static_cast<size_t>(real_bytes);
+}
+
+template <int N, typename R, bool REAL, bool F64, int KP>
+__global__ void __launch_bounds__(WaveCfg<N>::BS, (KP > 4 && WaveCfg<N>::BS == 64) ? 5 : 1)
     hilb_fast_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
+  using HT = typename std::conditional<REAL, R, C>::type;
   constexpr int N2 = N * N, BS = WaveCfg<N>::BS;
   __shared__ C s_rho[N2];
-  __shared__ C s_H[N2];
+  __shared__ HT s_H[N2];
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  R* s_vr = reinterpret_cast<R*>(s_dyn);  // real V: v_u,i at u*N + i
+  C* s_vc = reinterpret_cast<C*>(s_dyn);  // complex V
 
+  const auto bmp = cst(d.bmap) + blockIdx.x;
+  const BlockMap bm{bmp->g, bmp->l0, bmp->l1, 0};
   const Prologue pg = prologue(A);
   if (pg.skip) return;
   const int cur = pg.cur;
@@ -294,71 +320,97 @@ __global__ void __launch_bounds__(WaveCfg<N>::BS)
   const int ee = lane_ok ? e : 0;
   const int i = ee / N, j = ee - (ee / N) * N;
   const C* in = static_cast<const C*>(pick(A.in, cur));
-  const auto link_ptr = cst(d.link_ptr);
   const auto flat = cst(d.flat);
-  const auto gam = cst(d.gam);
 
-  // one ADO per workgroup; idle lanes (e >= N2) read element 0 and store nothing
-  const int g = blockIdx.x;
+  // idle lanes (e >= N2) read element 0 and store nothing
+  const int g = bm.g;
   const long long gi = static_cast<long long>(g) * N2 + ee;
+  const int nvd = (d.n_noise + 1) * N, nvd0 = d.n_noise * N;
+  const z128 hv = d.H[ee];  // BS >= N2: one element of H per lane
+  const z128 vdv = e < nvd0 ? d.Vd[e] : z128{0.0, 0.0};
+  const C dterm = conv<R>(d.dterm[ee]);
   const C r = in[gi];
+  const int l0 = bm.l0, l1 = bm.l1;
+  // the wide-operand stages issue their first four links before the barrier
+  // (measured: faster there, slower for stages of at most four operands)
+  const int nb0 = KP > 4 ? ((l1 - l0 < 4) ? l1 - l0 : 4) : 0;
+  LinkFlat q[4];
+  C x[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (t < nb0) {
+      q[t] = ld_flat(flat + l0 + t);
+      x[t] = in[static_cast<long long>(q[t].nbr) * N2 + ee];
+    }
+  }
   PreT<KP> pv;
   prefetch<F64, KP>(pv, A, cur, gi, true);
-  const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
-  const C gm = conv<R>(z128{gam[g].re, gam[g].im});
-  const C dterm = conv<R>(d.dterm[ee]);
-  for (int x2 = e; x2 < N2; x2 += BS) s_H[x2] = conv<R>(d.H[x2]);
+  const C gm = conv<R>(z128{cst(d.gam)[g].re, cst(d.gam)[g].im});
+  // LDS: H, the diagonals of V (zero row for direct links last), rho_n
+  if (lane_ok) {
+    if constexpr (REAL)
+      s_H[e] = static_cast<R>(hv.re);
+    else
+      s_H[e] = conv<R>(hv);
+  }
+  for (int xx = e; xx < nvd; xx += BS) {
+    const z128 v = xx == e ? vdv : (xx < nvd0 ? d.Vd[xx] : z128{0.0, 0.0});
+    if (REAL)
+      s_vr[xx] = static_cast<R>(v.re);
+    else
+      s_vc[xx] = conv<R>(v);
+  }
   if (lane_ok) s_rho[e] = r;
   __syncthreads();
 
   C acc{0, 0};
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    cmac(acc, s_H[i * N + k], s_rho[k * N + j]);
-    const C t = s_rho[i * N + k];
-    cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
+    const C b = s_rho[k * N + j], t = s_rho[i * N + k];
+    if constexpr (REAL) {
+      const R hik = s_H[i * N + k], hkj = s_H[k * N + j];
+      acc.re += hik * b.re;
+      acc.im += hik * b.im;
+      acc.re -= t.re * hkj;
+      acc.im -= t.im * hkj;
+    } else {
+      cmac(acc, s_H[i * N + k], b);
+      cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
+    }
   }
   C D = times_i(acc);
   cmac(D, gm + dterm, r);
-  const z128* cmt = d.cmac_tab;  // (n_noise+1) x N2 pairs (v_i - v_j, v_i + v_j)
-  for (int l = l0; l < l1; l += 4) {
+  // link l adds ((v_i - v_j) i*phi + (v_i + v_j) psi + dir) X_nbr, in link order
+  auto apply = [&](const LinkFlat& lq, const C& xv) {
+    C w;
+    if constexpr (REAL) {
+      const R vi = s_vr[lq.u * N + i], vj = s_vr[lq.u * N + j];
+      const R c0 = vi - vj, a0 = vi + vj;
+      w.re = c0 * static_cast<R>(lq.iphi.re) + a0 * static_cast<R>(lq.psi.re) +
+             static_cast<R>(lq.dir.re);
+      w.im = c0 * static_cast<R>(lq.iphi.im) + a0 * static_cast<R>(lq.psi.im) +
+             static_cast<R>(lq.dir.im);
+    } else {
+      const C vi = s_vc[lq.u * N + i], vj = s_vc[lq.u * N + j];
+      w = (vi - vj) * conv<R>(lq.iphi) + (vi + vj) * conv<R>(lq.psi) + conv<R>(lq.dir);
+    }
+    cmac(D, w, xv);
+  };
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (t < nb0) apply(q[t], x[t]);
+  for (int l = l0 + nb0; l < l1; l += 4) {
     const int nb = (l1 - l < 4) ? l1 - l : 4;
-    LinkFlat q[4];
-    C x[4];
-    z128 cm[4], ac[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       if (t < nb) {
         q[t] = ld_flat(flat + l + t);
         x[t] = in[static_cast<long long>(q[t].nbr) * N2 + ee];
-        const long long b = static_cast<long long>(q[t].u) * N2 + ee;
-        if (REALV) {
-          const z128 p = cmt[b];  // (cm, ac) as two reals
-          cm[t] = z128{p.re, 0.0};
-          ac[t] = z128{p.im, 0.0};
-        } else {
-          cm[t] = cmt[2 * b];
-          ac[t] = cmt[2 * b + 1];
-        }
       }
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      if (t < nb) {
-        C cf;
-        if (REALV) {
-          const R c0 = static_cast<R>(cm[t].re), a0 = static_cast<R>(ac[t].re);
-          cf.re = c0 * static_cast<R>(q[t].iphi.re) + a0 * static_cast<R>(q[t].psi.re) +
-                  static_cast<R>(q[t].dir.re);
-          cf.im = c0 * static_cast<R>(q[t].iphi.im) + a0 * static_cast<R>(q[t].psi.im) +
-                  static_cast<R>(q[t].dir.im);
-        } else {
-          cf = conv<R>(cm[t]) * conv<R>(q[t].iphi) + conv<R>(ac[t]) * conv<R>(q[t].psi) +
-               conv<R>(q[t].dir);
-        }
-        cmac(D, cf, x[t]);
-      }
-    }
+    for (int t = 0; t < 4; ++t)
+      if (t < nb) apply(q[t], x[t]);
   }
   double w2 = 0.0;
   if (lane_ok) {
@@ -853,7 +905,17 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
   const int t0 = wave * TPW;
   const int band = t0 / NB;
   double err_acc = 0.0;
-  for (int g = blockIdx.x; g < d.n_hrchy; g += gridDim.x) {
+  for (int w = blockIdx.x; w < d.n_hrchy; w += gridDim.x) {
+    // work unit w: the XCD-aware block map when present
+    BlockMap bm{w, 0, 0, 0};
+    if (d.bmap) {
+      const auto p = cst(d.bmap) + w;
+      bm = BlockMap{p->g, p->l0, p->l1, 0};
+    } else {
+      bm.l0 = link_ptr[w];
+      bm.l1 = link_ptr[w + 1];
+    }
+    const int g = bm.g;
     const long long base = static_cast<long long>(g) * n2;
     {
       // all loads of rho_n in flight before the LDS stores
@@ -921,7 +983,7 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
     constexpr int EPT = N2 / kMfmaBS;         // elements per thread (tile)
     constexpr int LCH = EPT < 8 ? EPT : 8;    // elements per load group
     constexpr int CH = KP <= 4 ? 2 : 1;
-    const int l0 = link_ptr[g], l1 = link_ptr[g + 1];
+    const int l0 = bm.l0, l1 = bm.l1;
     if constexpr (kLinksOuter) {
       for (int l = l0; l < l1; ++l) {
         const LinkFlat q = ld_flat(flat + l);
@@ -1830,36 +1892,39 @@ inline int cap_grid(long long n) {
 
 template <int N, typename R, bool F64>
 int launch_hilb_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
-  if (N >= 6 && d.diag_v && d.n_hrchy <= kMaxGrid) {
+  const bool real = d.real_v && d.real_h;
+  if (N >= 6 && d.diag_v && d.bmap != nullptr && d.n_hrchy <= kMaxGrid &&
+      fast_vd_lds(d, real, sizeof(R)) <= 32 * 1024) {
     constexpr int BS = WaveCfg<N>::BS;
     const int grid = d.n_hrchy;
+    const size_t lds = fast_vd_lds(d, real, sizeof(R));
     // four prefetched operands cover every stage except the last Dormand-Prince one
-    const bool small = A.npre <= 4 && !A.do_err;
-    const bool mid = A.npre <= 6 && !A.do_err;  // Dormand-Prince stages 5 and 6
-    if (d.real_v) {
+    const bool small = A.npre <= 4;
+    const bool mid = A.npre <= 6;  // Dormand-Prince stages 5 and 6
+    if (real) {
       if (small)
-        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, 4>), dim3(grid), dim3(BS), 0, st,
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, 4>), dim3(grid), dim3(BS), lds, st,
                            d, A);
       else if (mid)
-        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, 6>), dim3(grid), dim3(BS), 0, st,
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, 6>), dim3(grid), dim3(BS), lds, st,
                            d, A);
       else
-        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, kMaxPre>), dim3(grid), dim3(BS), 0,
-                           st, d, A);
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, true, F64, kMaxPre>), dim3(grid), dim3(BS),
+                           lds, st, d, A);
       return grid;
     } else {
       if (small)
-        hipLaunchKernelGGL((hilb_fast_kernel<N, R, false, F64, 4>), dim3(grid), dim3(BS), 0, st,
-                           d, A);
-      else
-        hipLaunchKernelGGL((hilb_fast_kernel<N, R, false, F64, kMaxPre>), dim3(grid), dim3(BS), 0,
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, false, F64, 4>), dim3(grid), dim3(BS), lds,
                            st, d, A);
+      else
+        hipLaunchKernelGGL((hilb_fast_kernel<N, R, false, F64, kMaxPre>), dim3(grid), dim3(BS),
+                           lds, st, d, A);
       return grid;
     }
   }
   constexpr int APB = HilbCfg<N>::APB, BS = HilbCfg<N>::BS;
   const int grid = cap_grid((d.n_hrchy + APB - 1) / APB);
-  const bool small = A.npre <= 4 && !A.do_err;
+  const bool small = A.npre <= 4;
   if (d.diag_v) {
     const size_t lds = sizeof(cx<R>) * (d.n_noise + 1) * N;
     if (lds > 48 * 1024)
@@ -1892,7 +1957,7 @@ template <int N, typename R, bool F64, bool BF16X3>
 int launch_hilb_mfma_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
   const size_t lds = 2 * sizeof(R) * N * MfmaCfg<N>::LD;
   const int grid = cap_grid(d.n_hrchy);
-  const bool small = A.npre <= 4 && !A.do_err;
+  const bool small = A.npre <= 4;
 #define HAMD_MFMA(REALH, KP, PAD)                                                \
   {                                                                              \
     auto k = hilb_mfma_kernel<N, R, REALH, F64, KP, BF16X3, PAD>;                \
@@ -1937,14 +2002,14 @@ int launch_hilb_big(const FormDev& d, const StageArgs& A, hipStream_t st) {
   const size_t one = sizeof(cx<R>) * static_cast<size_t>(d.N2);
   const int grid = cap_grid(d.n_hrchy);
   if (2 * one <= 64 * 1024) {
-    if (A.npre <= 4 && !A.do_err)
+    if (A.npre <= 4)
       hipLaunchKernelGGL((hilb_big_kernel<R, true, F64, 4>), dim3(grid), dim3(kBigBS), 2 * one,
                          st, d, A);
     else
       hipLaunchKernelGGL((hilb_big_kernel<R, true, F64, kMaxPre>), dim3(grid), dim3(kBigBS),
                          2 * one, st, d, A);
   } else if (one <= 64 * 1024) {
-    if (A.npre <= 4 && !A.do_err)
+    if (A.npre <= 4)
       hipLaunchKernelGGL((hilb_big_kernel<R, false, F64, 4>), dim3(grid), dim3(kBigBS), one, st,
                          d, A);
     else
@@ -1973,7 +2038,7 @@ int launch_hilb(const FormDev& d, const StageArgs& A, hipStream_t st) {
 
 template <typename R, bool F64>
 int launch_redfield(const FormDev& d, const StageArgs& A, hipStream_t st) {
-  const bool small = A.npre <= 4 && !A.do_err;
+  const bool small = A.npre <= 4;
   if (d.kind == HEOM_REDFIELD_LIOU) {
     const int grid = cap_grid((d.main_size + 3) / 4);  // four rows per workgroup
     if (small)
@@ -2015,7 +2080,7 @@ int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
     return launch_redfield<R, F64>(d, A, st);
   if (d.kind == kBatch) {
     const int grid = d.batch_grid > 0 ? d.batch_grid : 1;
-    if (A.npre <= 4 && !A.do_err)
+    if (A.npre <= 4)
       hipLaunchKernelGGL((batch_kernel<R, F64, 4>), dim3(grid), dim3(64), 0, st, d, A);
     else
       hipLaunchKernelGGL((batch_kernel<R, F64, kMaxPre>), dim3(grid), dim3(64), 0, st, d, A);
@@ -2024,14 +2089,14 @@ int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
   if (d.kind == 1) {
     const int APB = 256 / d.N2;
     const int grid = cap_grid((d.n_hrchy + APB - 1) / APB);
-    if (A.npre <= 4 && !A.do_err)
+    if (A.npre <= 4)
       hipLaunchKernelGGL((liou_kernel<R, F64, 4>), dim3(grid), dim3(256), 0, st, d, A);
     else
       hipLaunchKernelGGL((liou_kernel<R, F64, kMaxPre>), dim3(grid), dim3(256), 0, st, d, A);
     return grid;
   }
   const int grid = cap_grid((d.main_size + 255) / 256);
-  if (A.npre <= 4 && !A.do_err)
+  if (A.npre <= 4)
     hipLaunchKernelGGL((ado_kernel<R, F64, 4>), dim3(grid), dim3(256), 0, st, d, A);
   else
     hipLaunchKernelGGL((ado_kernel<R, F64, kMaxPre>), dim3(grid), dim3(256), 0, st, d, A);

@@ -332,6 +332,56 @@ void Qme::build_links() {
   }
 }
 
+// Workgroups are dealt to the eight XCDs round-robin, and the neighbours an
+// ADO reads are cheap only when they sit in the same XCD's L2.  ADOs are
+// therefore listed in depth-first preorder of the tree in which every ADO hangs
+// from the one that generated it in the breadth-first enumeration (n - e_k, k
+// its highest occupied mode: hrchy_space.cc:160-198 pushes n + e_k only for k
+// at or above the mode last raised), cut into eight consecutive parts of equal
+// size (the ADOs and their links then mix the tiers evenly), and each part is
+// given back in breadth-first order to workgroups x, x + 8, x + 16, ...
+// Subtrees stay together, so most +-1 links join ADOs of one part.
+std::vector<BlockMap> Qme::build_block_map() const {
+  const int nh = hs.n_hrchy, K = hs.n_dim;
+  constexpr int P = 8;
+  std::vector<int> order;
+  order.reserve(nh);
+  std::vector<char> seen(nh, 0);
+  std::vector<int> stack{0};
+  while (!stack.empty()) {
+    const int g = stack.back();
+    stack.pop_back();
+    if (g < 0 || g >= nh || seen[g]) continue;
+    seen[g] = 1;
+    order.push_back(g);
+    const int* ng = &hs.n[static_cast<size_t>(g) * K];
+    int top = 0;
+    for (int k = 0; k < K; ++k)
+      if (ng[k] > 0) top = k;
+    for (int k = K - 1; k >= top; --k) {  // lower modes are visited first
+      const int c = hs.ptr_p1[static_cast<size_t>(g) * K + k];
+      if (c != nh) stack.push_back(c);
+    }
+  }
+  for (int g = 0; g < nh; ++g)  // anything the tree walk missed keeps its order
+    if (!seen[g]) order.push_back(g);
+  std::vector<std::vector<int>> part(P);
+  size_t pos = 0;
+  for (int x = 0; x < P; ++x) {
+    const int cnt = (nh - x + P - 1) / P;  // workgroups b < nh with b % 8 == x
+    part[x].assign(order.begin() + pos, order.begin() + pos + cnt);
+    std::sort(part[x].begin(), part[x].end());
+    pos += cnt;
+  }
+  std::vector<BlockMap> map(nh);
+  for (int x = 0; x < P; ++x)
+    for (size_t i = 0; i < part[x].size(); ++i) {
+      const int g = part[x][i];
+      map[i * P + x] = BlockMap{g, link_ptr[g], link_ptr[g + 1], 0};
+    }
+  return map;
+}
+
 void Qme::build_superops(std::vector<int>& ptr, std::vector<int>& col, std::vector<z128>& val,
                          std::vector<std::vector<std::map<int, z128>>>& ops) {
   const int nn = static_cast<int>(noise_work_.size());
@@ -469,6 +519,7 @@ void Qme::upload() {
   HAMD_HIP(hipSetDevice(opt.device));
   d_link_ptr.upload(link_ptr);
   d_links.upload(links);
+  d_bmap.upload(build_block_map());
   std::vector<LinkFlat> flat(links.size());
   for (size_t l = 0; l < links.size(); ++l) {
     const Coef& cf = coef[links[l].cid];
@@ -563,6 +614,7 @@ void Qme::upload() {
   dev.diag_v = diag_v ? 1 : 0;
   dev.real_v = real_v ? 1 : 0;
   dev.flat = d_flat.as<LinkFlat>();
+  dev.bmap = d_bmap.as<BlockMap>();
   dev.cmac_tab = d_cmac.as<z128>();
   dev.max_links = 0;
   for (int l = 0; l < hs.n_hrchy; ++l)

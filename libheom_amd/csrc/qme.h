@@ -89,6 +89,7 @@ class Qme {
   FormDev dev{};
   hipStream_t stream = nullptr;
   DevBuf d_link_ptr, d_links, d_flat, d_cmac, d_coef, d_gam, d_H, d_V, d_Vd, d_dterm, d_sdelta;
+  DevBuf d_bmap;
   DevBuf d_sop_ptr, d_sop_col, d_sop_val;
   DevBuf d_R_ptr, d_R_col, d_R_val;
   long long ado_nnz = 0;
@@ -139,6 +140,8 @@ class Qme {
   std::vector<z128> R_val_;
 
   void build_links();
+  // Work-unit order of the one-ADO-per-workgroup kernels (see BlockMap)
+  std::vector<BlockMap> build_block_map() const;
   void set_param_redfield();
   void build_superops(std::vector<int>& ptr, std::vector<int>& col, std::vector<z128>& val,
                       std::vector<std::vector<std::map<int, z128>>>& ops);

@@ -19,7 +19,8 @@ namespace {
 // slot numbers
 enum {
   SL_Y = 0, SL_YNEW = 1, SL_Y32 = 2, SL_YNEW32 = 3,
-  SL_SA = 4, SL_SB = 5, SL_ACC = 6, SL_K1 = 7  // K1..K6 = 7..12
+  SL_SA = 4, SL_SB = 5, SL_ACC = 6, SL_K1 = 7,  // K1..K5 = 7..11
+  SL_E = 13  // Dormand-Prince: sum_{j<6} E_j k_j, always FP64
 };
 }  // namespace
 
@@ -80,7 +81,8 @@ Solver::Solver(std::shared_ptr<Qme> q_, int kind_)
   if (kind == HEOM_RK4 || kind == HEOM_LSRK4) {
     mk(SL_ACC, z128b, 0);  // rk4: the accumulator; lsrk4: buff
   } else {
-    for (int k = 0; k < 6; ++k) mk(SL_K1 + k, sinb, lo);
+    for (int k = 0; k < 5; ++k) mk(SL_K1 + k, sinb, lo);
+    mk(SL_E, z128b, 0);
   }
   if (q->opt.renorm_rho0) {
     if (q->kind == kBatch)
@@ -212,37 +214,42 @@ void Solver::build_stages(double atol, double rtol) {
   } else {
     // Dormand-Prince: stage i (0-based) evaluates k_i at its input and writes
     // the input of stage i+1 (rkdp.h:111-127); stage 5 also forms the new
-    // state (rkdp.h:130-132); stage 6 evaluates k_7 at the new state and the
-    // error norm (rkdp.h:136-153).
+    // state (rkdp.h:130-132) and, from the same operands, the error sum
+    // e6 = sum_{j<6} E_j k_j in the order of rkdp.h:142-145 (k_6 itself is
+    // not stored); stage 6 evaluates k_7 at the new state, completes
+    // e = e6 + E_7 k_7 and the error norm (rkdp.h:144-153).  The error vector
+    // is the same fma chain as summing all seven terms in the last stage.
     const int sin_[6] = {y_in, SL_SA, SL_SB, SL_SA, SL_SB, SL_SA};
     const int sout[5] = {SL_SA, SL_SB, SL_SA, SL_SB, SL_SA};
     for (int i = 0; i < 6; ++i) {
       StageSpec a;
       a.in_slot = sin_[i];
-      a.k_slot = SL_K1 + i;
       std::vector<std::pair<int, double>> terms{{SL_Y, 1.0}};
       if (i < 5) {
+        a.k_slot = SL_K1 + i;
         for (int j = 0; j < i; ++j) terms.push_back({SL_K1 + j, kDpA[i + 1][j]});
         a.combo(terms, kDpA[i + 1][i], sout[i]);
       } else {
         for (int j = 0; j < 5; ++j)
           if (kDpB[j] != 0.0) terms.push_back({SL_K1 + j, kDpB[j]});
         a.combo(terms, kDpB[5], SL_YNEW, lo ? SL_YNEW32 : -1);
+        std::vector<std::pair<int, double>> eterms;
+        for (int j = 0; j < 5; ++j)
+          if (kDpE[j] != 0.0) eterms.push_back({SL_K1 + j, kDpE[j]});
+        a.combo(eterms, kDpE[5], SL_E);
       }
       specs.push_back(a);
     }
     StageSpec a;
     a.in_slot = lo ? SL_YNEW32 : SL_YNEW;
     a.do_err = 1;
-    for (int j = 0; j < 6; ++j)
-      if (kDpE[j] != 0.0) a.ec[a.pre_index(SL_K1 + j)] = kDpE[j];
+    a.ec[a.pre_index(SL_E)] = 1.0;
     a.ek = kDpE[6];
     // the error weights read y_new / y_old from the last two operand positions
-    while (static_cast<int>(a.pre.size()) < kMaxPre - 2) a.pre.push_back(-1);
     a.pre.push_back(SL_YNEW);
     a.pre.push_back(SL_Y);
-    a.ynew_pre = kMaxPre - 2;
-    a.yold_pre = kMaxPre - 1;
+    a.ynew_pre = 1;
+    a.yold_pre = 2;
     a.control = 1;
     specs.push_back(a);
   }

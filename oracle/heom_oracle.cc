@@ -205,6 +205,9 @@ struct Model {
   int n_threads = 1;
   bool parity_q1 = true;   // reference quirk: off-diagonal gamma dropped
   bool sqrt_norm = false;  // LIBHEOM_SQRT_NORMALIZATION variant
+  // sensitivity experiments only: accumulate the per-noise coupling terms of
+  // heom_hilb in reverse noise order (the same operator with other rounding)
+  bool reverse_noise_order = false;
   vector<cplx> H;
   vector<char> Hnz;
   vector<Noise> noise;
@@ -695,7 +698,8 @@ void calc_diff_hilb(Model& m, cplx* drho, const cplx* rho, cplx alpha, cplx beta
       gemm(N, m.noise[u].s_delta, V, tPhi, 1.0, d);
       gemm(N, -m.noise[u].s_delta, tPhi, V, 1.0, d);
     }
-    for (int u = 0; u < nn; ++u) {
+    for (int uu = 0; uu < nn; ++uu) {
+      const int u = m.reverse_noise_order ? nn - 1 - uu : uu;
       const auto& lk_u = m.lk[u];
       const int len = m.noise[u].len;
       const cplx* V = m.noise[u].V.data();
@@ -764,7 +768,8 @@ void calc_diff_liou(Model& m, cplx* drho, const cplx* rho, cplx alpha, cplx beta
     cplx* tPsi = &m.temp[static_cast<size_t>(3 * tid + 2) * N2];
     supop_gemv(1.0, m.R0, rho_n, 0.0, d);
     axpy(N2, m.ngamma_diag[l], rho_n, d);
-    for (int u = 0; u < nn; ++u) {
+    for (int uu = 0; uu < nn; ++uu) {
+      const int u = m.reverse_noise_order ? nn - 1 - uu : uu;
       const auto& lk_u = m.lk[u];
       const int len = m.noise[u].len;
       for (auto& t : m.gamma_offdiag[u]) {
@@ -991,6 +996,12 @@ void* orc_create(int kind, int n_level, int max_depth, int n_threads) {
 }
 
 void orc_destroy(void* p) { delete static_cast<Model*>(p); }
+
+// Sensitivity experiments: reverse == 1 sums the heom_hilb coupling terms of
+// the noises in reverse order (a reordering of the same arithmetic).
+void orc_set_sum_order(void* p, int reverse) {
+  static_cast<Model*>(p)->reverse_noise_order = reverse != 0;
+}
 
 void orc_set_filter(void* p, int (*filter)(const int*, int, int, void*), void* user) {
   auto* m = static_cast<Model*>(p);

@@ -58,6 +58,7 @@ def lib():
         L.orc_set_options.argtypes = [vp, ip, ip]
         L.orc_set_filter.argtypes = [vp, FILTER, vp]
         L.orc_set_threads.argtypes = [vp, ip]
+        L.orc_set_sum_order.argtypes = [vp, ip]
         L.orc_set_H.argtypes = [vp, vp]
         L.orc_alloc_noises.argtypes = [vp, ip]
         L.orc_set_noise.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp, vp, dp, dp]
@@ -136,6 +137,12 @@ class Oracle:
 
     def set_threads(self, n):
         self._L.orc_set_threads(self.h, n)
+
+    def set_sum_order(self, reverse):
+        """Sensitivity experiments: sum the heom_hilb coupling terms of the
+        noises in reverse order (the same operator, other rounding)."""
+        self._L.orc_set_sum_order(self.h, int(bool(reverse)))
+        return self
 
     def set_filter(self, fn):
         """hrchy_filter(index ndarray, depth) -> bool (hrchy_space.h:29-36); before load()."""

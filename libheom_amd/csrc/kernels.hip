@@ -735,31 +735,26 @@ __device__ __forceinline__ mf_f4 mac3(const Split8& a, const Split8& b, mf_f4 c)
 }
 
 constexpr int kMfmaBS = 256;
+constexpr size_t kMfmaLdsCap = 150 * 1024;  // dynamic LDS the matrix-core kernel may use
 
-// The link coefficient table entries of element tb of one noise: (cm, ac) as
-// two reals for real V, else the complex pair.
-__device__ __forceinline__ void link_coef_load(const FormDev& d, long long tb, z128& p0,
-                                               z128& p1) {
-  if (d.real_v) {
-    p0 = d.cmac_tab[tb];
-  } else {
-    p0 = d.cmac_tab[2 * tb];
-    p1 = d.cmac_tab[2 * tb + 1];
-  }
-}
-
-// Coefficient of a neighbour element: cm*iphi + ac*psi + dir (diagonal V).
+// Coefficient of a neighbour element at (i, j) with the diagonals of every V_u
+// in LDS (sv: n_noise + 1 rows of n, the last row zero; real V as R, complex V
+// as cx<R>): (v_i - v_j)*iphi + (v_i + v_j)*psi + dir.
 template <typename R>
-__device__ __forceinline__ cx<R> link_coef(const FormDev& d, const LinkFlat& q, const z128& p0,
-                                           const z128& p1) {
+__device__ __forceinline__ cx<R> link_coef_lds(const FormDev& d, const LinkFlat& q, const R* sv,
+                                               int n, int i, int j) {
   cx<R> cf;
   if (d.real_v) {
-    cf.re = static_cast<R>(p0.re) * static_cast<R>(q.iphi.re) +
-            static_cast<R>(p0.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
-    cf.im = static_cast<R>(p0.re) * static_cast<R>(q.iphi.im) +
-            static_cast<R>(p0.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
+    const R vi = sv[q.u * n + i], vj = sv[q.u * n + j];
+    const R c0 = vi - vj, a0 = vi + vj;
+    cf.re = c0 * static_cast<R>(q.iphi.re) + a0 * static_cast<R>(q.psi.re) +
+            static_cast<R>(q.dir.re);
+    cf.im = c0 * static_cast<R>(q.iphi.im) + a0 * static_cast<R>(q.psi.im) +
+            static_cast<R>(q.dir.im);
   } else {
-    cf = conv<R>(p0) * conv<R>(q.iphi) + conv<R>(p1) * conv<R>(q.psi) + conv<R>(q.dir);
+    const cx<R>* sc = reinterpret_cast<const cx<R>*>(sv);
+    const cx<R> vi = sc[q.u * n + i], vj = sc[q.u * n + j];
+    cf = (vi - vj) * conv<R>(q.iphi) + (vi + vj) * conv<R>(q.psi) + conv<R>(q.dir);
   }
   return cf;
 }
@@ -887,14 +882,27 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
   extern __shared__ __align__(16) unsigned char s_dyn[];
   R* s_re = reinterpret_cast<R*>(s_dyn);
   R* s_im = s_re + N * LD;
+  R* s_v = s_im + N * LD;  // diagonals of every V_u (and a zero row), see link_coef_lds
   const Prologue pg = prologue(A);
   if (pg.skip) return;
   // the ADO block (n <= N); unpadded blocks keep the index arithmetic constant
   const int n = PADDED ? d.N : N, n2 = PADDED ? d.N2 : N * N;
   if (PADDED) {  // the padding stays zero: only rows and columns < n are ever written
     for (int x = threadIdx.x; x < 2 * N * LD; x += kMfmaBS) s_re[x] = R(0);
-    __syncthreads();
   }
+  {
+    const int nvd = (d.n_noise + 1) * d.N, nvd0 = d.n_noise * d.N;
+    for (int x = threadIdx.x; x < nvd; x += kMfmaBS) {
+      const z128 v = x < nvd0 ? d.Vd[x] : z128{0.0, 0.0};
+      if (d.real_v) {
+        s_v[x] = static_cast<R>(v.re);
+      } else {
+        s_v[2 * x] = static_cast<R>(v.re);
+        s_v[2 * x + 1] = static_cast<R>(v.im);
+      }
+    }
+  }
+  __syncthreads();
   const int cur = pg.cur;
   const C* in = static_cast<const C*>(pick(A.in, cur));
   const auto link_ptr = cst(d.link_ptr);
@@ -988,17 +996,14 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
       for (int l = l0; l < l1; ++l) {
         const LinkFlat q = ld_flat(flat + l);
         const C* nb = in + static_cast<long long>(q.nbr) * n2;
-        const long long tu = static_cast<long long>(q.u) * n2;
 #pragma unroll 1
         for (int c0 = 0; c0 < EPT; c0 += LCH) {
           C xv[LCH];
-          z128 p0[LCH], p1[LCH];
 #pragma unroll
           for (int c = 0; c < LCH; ++c) {
             const int x = threadIdx.x + (c0 + c) * kMfmaBS;
             const int xs = x < n2 ? x : 0;
             xv[c] = nb[xs];
-            link_coef_load(d, tu + xs, p0[c], p1[c]);
           }
 #pragma unroll
           for (int c = 0; c < LCH; ++c) {
@@ -1006,7 +1011,7 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
             if (x < n2) {
               const int i = x / n, j = x - (x / n) * n;
               C Dv{s_re[i * LD + j], s_im[i * LD + j]};
-              cmac(Dv, link_coef<R>(d, q, p0[c], p1[c]), xv[c]);
+              cmac(Dv, link_coef_lds<R>(d, q, s_v, d.N, i, j), xv[c]);
               s_re[i * LD + j] = Dv.re;
               s_im[i * LD + j] = Dv.im;
             }
@@ -1030,14 +1035,13 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
         for (int l = l0; l < l1; ++l) {
           const LinkFlat q = ld_flat(flat + l);
           C xv[CH];
-          z128 p0[CH], p1[CH];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) xv[c] = in[static_cast<long long>(q.nbr) * n2 + xs[c]];
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
-            xv[c] = in[static_cast<long long>(q.nbr) * n2 + xs[c]];
-            link_coef_load(d, static_cast<long long>(q.u) * n2 + xs[c], p0[c], p1[c]);
+            const int i = xs[c] / n, j = xs[c] - (xs[c] / n) * n;
+            cmac(Dc[c], link_coef_lds<R>(d, q, s_v, d.N, i, j), xv[c]);
           }
-#pragma unroll
-          for (int c = 0; c < CH; ++c) cmac(Dc[c], link_coef<R>(d, q, p0[c], p1[c]), xv[c]);
         }
       }
 #pragma unroll
@@ -1955,13 +1959,17 @@ void allow_lds(K kernel, size_t bytes) {
 
 template <int N, typename R, bool F64, bool BF16X3>
 int launch_hilb_mfma_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
-  const size_t lds = 2 * sizeof(R) * N * MfmaCfg<N>::LD;
+  // rho_n planes, then the diagonals of every V_u and a zero row
+  const size_t lds = 2 * sizeof(R) * N * MfmaCfg<N>::LD +
+                     sizeof(R) * (d.real_v ? 1 : 2) * static_cast<size_t>(d.n_noise + 1) * d.N;
+  if (lds > kMfmaLdsCap)
+    throw HeomError{HEOM_ERR_UNSUPPORTED, "matrix-core path: too many noises for LDS"};
   const int grid = cap_grid(d.n_hrchy);
   const bool small = A.npre <= 4;
 #define HAMD_MFMA(REALH, KP, PAD)                                                \
   {                                                                              \
     auto k = hilb_mfma_kernel<N, R, REALH, F64, KP, BF16X3, PAD>;                \
-    static bool once = (allow_lds(k, lds), true);                                \
+    static bool once = (allow_lds(k, kMfmaLdsCap), true);                        \
     (void)once;                                                                  \
     hipLaunchKernelGGL(k, dim3(grid), dim3(kMfmaBS), lds, st, d, A);             \
   }

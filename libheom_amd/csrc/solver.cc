@@ -153,8 +153,8 @@ StageArgs Solver::resolve(const StageSpec& s) const {
   a.yold_pre = s.yold_pre;
   a.control = s.control;
   a.inv_n = 1.0 / static_cast<double>(ms);
-  if (kind == HEOM_RKDP) a.ctrl = d_ctrl.as<Ctrl>();
-  a.partials = d_partials.as<double>();
+  if (kind == HEOM_RKDP) a.ctrl = ctrl_dev();
+  a.partials = partials_dev();
   a.max_partials = kMaxGrid;
   return a;
 }
@@ -327,7 +327,7 @@ void Solver::renorm(bool with_ctrl) {
   launch_rho0_renorm(slot_ptr(SL_Y, 0), slot_ptr(SL_Y, with_ctrl ? 1 : 0),
                      lo ? slot_ptr(SL_Y32, 0) : nullptr,
                      lo ? slot_ptr(SL_Y32, with_ctrl ? 1 : 0) : nullptr,
-                     with_ctrl ? d_ctrl.as<Ctrl>() : nullptr, q->N, d_tr0.as<double>(), q->stream);
+                     with_ctrl ? ctrl_dev() : nullptr, q->N, d_tr0.as<double>(), q->stream);
 }
 
 void Solver::launch_stage(int s, double dt) {
@@ -448,7 +448,7 @@ void Solver::rk4_interval(double t0, double t1, double dt) {
 // adaptive_step_size_solver::solve_1 (adaptive_step_size_solver.h:32-56) with
 // the attempts of rkdp::solve_adaptive_step launched in batches.
 int Solver::rkdp_interval(double t0, double t1, double dt_first, int& batch) {
-  launch_ctrl_init(d_ctrl.as<Ctrl>(), t0, t1, dt_first, q->stream);
+  launch_ctrl_init(ctrl_dev(), t0, t1, dt_first, q->stream);
   long long a0 = h_ctrl->attempts;
   // the projection kernel runs after every attempt and acts only when the
   // control block counts a new accepted step
@@ -462,13 +462,22 @@ int Solver::rkdp_interval(double t0, double t1, double dt_first, int& batch) {
     } else {
       for (int b = 0; b < batch; ++b) attempt();
     }
-    HAMD_HIP(hipMemcpyAsync(h_ctrl, d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, q->stream));
+    HAMD_HIP(hipMemcpyAsync(h_ctrl, ctrl_dev(), sizeof(Ctrl), hipMemcpyDeviceToHost, q->stream));
     HAMD_HIP(hipStreamSynchronize(q->stream));
     if (h_ctrl->done) break;
-    batch = std::min(kMaxAttemptBatch, std::max(1, batch * 2));
+    // attempts still needed at the current step (attempts after the interval
+    // is done are no-ops of a few microseconds each; a short batch costs one
+    // more host round trip)
+    const double rem = (h_ctrl->t_bound - h_ctrl->t) / h_ctrl->dt;
+    const long long est =
+        (rem > 0.0 && rem < static_cast<double>(kMaxAttemptBatch)) ? static_cast<long long>(
+                                                                         std::ceil(rem))
+                                                                   : 2LL * batch;
+    batch = static_cast<int>(std::min<long long>(kMaxAttemptBatch, std::max<long long>(1, est)));
   }
-  long long used = h_ctrl->attempts - a0;
-  batch = static_cast<int>(std::max<long long>(1, std::min<long long>(used + 1, 4096)));
+  // the next interval starts with as many attempts as this one took
+  const long long used = h_ctrl->attempts - a0;
+  batch = static_cast<int>(std::max<long long>(1, std::min<long long>(used, 4096)));
   return h_ctrl->gave_up;
 }
 
@@ -506,7 +515,7 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
     HAMD_HIP(hipMemcpyAsync(rho_dev, P[SL_Y].p, bytes, hipMemcpyDeviceToDevice, st));
   } else {
     std::memset(h_ctrl, 0, sizeof(Ctrl));
-    HAMD_HIP(hipMemcpyAsync(d_ctrl.p, h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, st));
+    HAMD_HIP(hipMemcpyAsync(ctrl_dev(), h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, st));
     HAMD_HIP(hipStreamSynchronize(st));
     double first = dt_save > 0.0 ? dt_save : dt;
     int batch = 4;

@@ -85,6 +85,8 @@ class Solver {
   void* sp[kSlots] = {};
   int slo[kSlots] = {};  // 1: complex64 buffer
   DevBuf d_ctrl, d_partials;
+  Ctrl* ctrl_dev() const { return d_ctrl.as<Ctrl>(); }
+  double* partials_dev() const { return d_partials.as<double>(); }
   DevBuf d_tr0;  // Tr rho_0 at the start of the solve (renorm_rho0)
   void renorm(bool with_ctrl);
   Ctrl* h_ctrl = nullptr;  // pinned mirror

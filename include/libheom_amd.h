@@ -243,6 +243,21 @@ int heom_solve(heom_solver_handle s, double* rho_host, const double* t_list, int
  * place; rho_s_out (host, n_t x N*N complex128) receives rho_S at every t_i. */
 int heom_solve_dev(heom_solver_handle s, void* rho_dev, const double* t_list, int n_t,
                    double dt, double atol, double rtol, double* rho_s_out);
+/* No counterpart in the reference: heom_solve_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * heom_solve_dev with its stage kernels launched directly (no graph replay, no
+ * resident integrator), and no hipMalloc / hipFree / graph instantiation on the
+ * call: the Dormand-Prince control block, the error-norm partials and rho_S of
+ * every output time live in one fixed 8 MiB device array per GPU, held until
+ * the process ends.  Limit: n_t * n_level^2 * 16 bytes of rho_S <= 6 MiB (for a
+ * batch, n_level^2 is the sum over its members); larger requests return
+ * HEOM_ERR_ARG.  The solver's own state buffers come from heom_solver_create.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (from
+ * several host threads or streams) race; calls in sequence are independent.
+ * HEOM_ERR_UNSUPPORTED when a closure network is bound. */
+int heom_solve_dev_noalloc(heom_solver_handle s, void* rho_dev, const double* t_list, int n_t,
+                           double dt, double atol, double rtol, double* rho_s_out);
 int heom_solver_stats(heom_solver_handle s, heom_stats* out);
 int heom_solver_reset(heom_solver_handle s); /* forget dt_save (adaptive_step_size_solver.h:28) */
 /* 1 (default): replay the fixed stage sequences (RK4 steps of the full dt,

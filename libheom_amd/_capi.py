@@ -30,7 +30,7 @@ EXPORTS = [
     "heom_set_H", "heom_alloc_noises", "heom_set_noise", "heom_set_hrchy_filter", "heom_set_param", "heom_get_n_hrchy",
     "heom_get_n_dim", "heom_main_size", "heom_get_hrchy", "heom_redfield_lambda", "heom_calc_diff",
     "heom_calc_diff_host", "heom_batch_create", "heom_batch_layout", "heom_solver_create", "heom_solver_destroy", "heom_solve",
-    "heom_solve_dev", "heom_solver_stats", "heom_solver_reset", "heom_solver_set_graphs", "heom_solver_time_stages",
+    "heom_solve_dev", "heom_solve_dev_noalloc", "heom_solver_stats", "heom_solver_reset", "heom_solver_set_graphs", "heom_solver_time_stages",
     "heom_solver_stage_bytes", "heom_dev_alloc", "heom_dev_free", "heom_memcpy_h2d",
     "heom_memcpy_d2h", "heom_device_sync", "heom_stream_copy", "heom_mlp_create", "heom_mlp_destroy",
     "heom_mlp_n_params", "heom_mlp_get_params", "heom_mlp_set_params", "heom_mlp_set_scaling",
@@ -106,6 +106,7 @@ def load():
         "heom_solver_destroy": (ip, [vp]),
         "heom_solve": (ip, [vp, vp, vp, ip, CALLBACK, vp, dp, dp, dp, ip]),
         "heom_solve_dev": (ip, [vp, vp, vp, ip, dp, dp, dp, vp]),
+        "heom_solve_dev_noalloc": (ip, [vp, vp, vp, ip, dp, dp, dp, vp]),
         "heom_solver_stats": (ip, [vp, P(heom_stats)]),
         "heom_solver_reset": (ip, [vp]),
         "heom_solver_set_graphs": (ip, [vp, ip]),

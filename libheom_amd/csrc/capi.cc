@@ -9,6 +9,7 @@
 #include "kernels.h"
 #include "qme.h"
 #include "mlp.h"
+#include "noalloc.h"
 #include "solver.h"
 
 // The QME is shared with the solvers created on it: a solver keeps it alive
@@ -517,6 +518,34 @@ int heom_solve_dev(heom_solver_handle s, void* rho_dev, const double* t_list, in
     if (rho_s_out)
       HAMD_HIP(hipMemcpy(rho_s_out, rs.p, sizeof(hamd::z128) * N2 * static_cast<size_t>(n_t),
                          hipMemcpyDeviceToHost));
+    if (rc != HEOM_OK)
+      hamd::set_error(rc, "rkdp: step rejected 1000 times; interval skipped as in the reference");
+    return rc;
+  });
+}
+
+int heom_solve_dev_noalloc(heom_solver_handle s, void* rho_dev, const double* t_list, int n_t,
+                           double dt, double atol, double rtol, double* rho_s_out) {
+  return guard([&] {
+    if (!s || !s->s) throw hamd::HeomError{HEOM_ERR_ARG, "null solver"};
+    if (!rho_dev || !t_list) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solve_dev_noalloc: NULL"};
+    if (n_t < 1) throw hamd::HeomError{HEOM_ERR_ARG, "heom_solve_dev_noalloc: n_t < 1"};
+    hamd::Solver& sv = *s->s;
+    sv.check_qme();
+    const bool batch = sv.q->kind == hamd::kBatch;
+    const size_t N2 = static_cast<size_t>(batch ? sv.q->batch_rs_size : sv.q->N2);
+    const size_t rs_bytes = sizeof(hamd::z128) * N2 * static_cast<size_t>(n_t);
+    if (rs_bytes > hamd::kNoallocRhoSBytes)
+      throw hamd::HeomError{HEOM_ERR_ARG,
+                            "heom_solve_dev_noalloc: n_t x n_level^2 x 16 bytes of rho_S exceed "
+                            "the scratch (" + std::to_string(hamd::kNoallocRhoSBytes) +
+                                " bytes)"};
+    char* base = static_cast<char*>(hamd::noalloc_scratch(sv.q->opt.device));
+    auto* ctrl = reinterpret_cast<hamd::Ctrl*>(base + hamd::kNoallocCtrlOff);
+    auto* partials = reinterpret_cast<double*>(base + hamd::kNoallocPartialsOff);
+    void* rs = base + hamd::kNoallocRhoSOff;
+    int rc = sv.solve_noalloc(rho_dev, t_list, n_t, dt, atol, rtol, ctrl, partials, rs);
+    if (rho_s_out) HAMD_HIP(hipMemcpy(rho_s_out, rs, rs_bytes, hipMemcpyDeviceToHost));
     if (rc != HEOM_OK)
       hamd::set_error(rc, "rkdp: step rejected 1000 times; interval skipped as in the reference");
     return rc;

@@ -497,8 +497,8 @@ int Solver::solve(void* rho_dev, const double* t_list, int n_t, double dt, doubl
   }
   HAMD_HIP(hipSetDevice(q->opt.device));
   build_stages(atol, rtol);
-  build_resident();
-  reset_graphs();  // stage arguments (tolerances, closure) may have changed
+  build_resident();  // nothing without graphs
+  if (!noalloc) reset_graphs();  // stage arguments (tolerances, closure) may have changed
   const size_t bytes = sizeof(z128) * ms;
   hipStream_t st = q->stream;
   // the caller's state enters the FP64 state slot
@@ -651,6 +651,41 @@ int Solver::solve_resident_dopri(void* rho_dev, const double* t_list, int n_t, d
   stats.ado_stage_updates = stats.n_rhs * q->hs.n_hrchy;
   stats.gave_up |= h_ctrl->gave_up;
   return h_ctrl->gave_up ? HEOM_ERR_STEP_REJECTED : HEOM_OK;
+}
+
+int Solver::solve_noalloc(void* rho_dev, const double* t_list, int n_t, double dt, double atol,
+                          double rtol, Ctrl* ctrl, double* partials, void* rho_s_dev) {
+  check_qme();
+  if (q->closure) throw HeomError{HEOM_ERR_UNSUPPORTED, "solve_noalloc: a closure is bound"};
+  const bool batch = q->kind == kBatch;
+  const size_t rs_bytes = sizeof(z128) * (batch ? q->batch_rs_size : q->N2);
+  hipStream_t st = q->stream;
+  const bool graphs = use_graphs;
+  use_graphs = false;
+  noalloc = true;
+  ext_ctrl = ctrl;
+  ext_partials = partials;
+  int rc = HEOM_OK;
+  try {
+    rc = solve(rho_dev, t_list, n_t, dt, atol, rtol, [&](int i, const void* cur) {
+      char* dst = static_cast<char*>(rho_s_dev) + rs_bytes * i;
+      if (batch)
+        launch_gather_rho_s(q->dev, dst, cur, st);
+      else
+        HAMD_HIP(hipMemcpyAsync(dst, cur, rs_bytes, hipMemcpyDeviceToDevice, st));
+    });
+  } catch (...) {
+    use_graphs = graphs;
+    noalloc = false;
+    ext_ctrl = nullptr;
+    ext_partials = nullptr;
+    throw;
+  }
+  use_graphs = graphs;
+  noalloc = false;
+  ext_ctrl = nullptr;
+  ext_partials = nullptr;
+  return rc;
 }
 
 void Solver::time_stages(int reps, std::vector<double>& out) {

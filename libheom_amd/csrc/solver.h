@@ -69,6 +69,14 @@ class Solver {
   int solve_gather(void* rho_dev, const double* t_list, int n_t, double dt, double atol,
                    double rtol, void* rho_s_dev);
 
+  // solve_gather without any device allocation, for callers that cannot
+  // allocate: every stage kernel is launched directly (no graphs, no resident
+  // integrator), the control block and the error-norm partials are the
+  // caller's (ctrl, partials: kMaxGrid doubles), and rho_S of every output time
+  // is copied to rho_s_dev.  No closure may be bound.
+  int solve_noalloc(void* rho_dev, const double* t_list, int n_t, double dt, double atol,
+                    double rtol, Ctrl* ctrl, double* partials, void* rho_s_dev);
+
   // Average duration (ms) of each stage kernel over `reps` replays.
   void time_stages(int reps, std::vector<double>& ms_out);
   // throws HEOM_ERR_STATE unless the QME is set up with the size the solver's
@@ -85,8 +93,13 @@ class Solver {
   void* sp[kSlots] = {};
   int slo[kSlots] = {};  // 1: complex64 buffer
   DevBuf d_ctrl, d_partials;
-  Ctrl* ctrl_dev() const { return d_ctrl.as<Ctrl>(); }
-  double* partials_dev() const { return d_partials.as<double>(); }
+  // control block and partials of the current solve (the solver's own, or a
+  // caller's in solve_noalloc); noalloc: no graph, upload or resident launch
+  Ctrl* ext_ctrl = nullptr;
+  double* ext_partials = nullptr;
+  bool noalloc = false;
+  Ctrl* ctrl_dev() const { return ext_ctrl ? ext_ctrl : d_ctrl.as<Ctrl>(); }
+  double* partials_dev() const { return ext_partials ? ext_partials : d_partials.as<double>(); }
   DevBuf d_tr0;  // Tr rho_0 at the start of the solve (renorm_rho0)
   void renorm(bool with_ctrl);
   Ctrl* h_ctrl = nullptr;  // pinned mirror

@@ -38,29 +38,46 @@ def mk_solver(H, eng, q, method):
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hilb_fast_kernel_7site_traffic.json")
 
 
-def summarize_pmc(fetch_csv, write_csv, state_bytes):
+def _pmc_rows(path, counter):
+    """(kernel name, grid size, value) of every dispatch in a rocprofv3 counter
+    run: its SQLite results file (*_results.db) or counter_collection.csv."""
+    if path.endswith(".db"):
+        import sqlite3
+        con = sqlite3.connect(path)
+        q = ("select kernel_name, grid_size, value from counters_collection "
+             "where counter_name = ?")
+        return [(k, int(g), float(v)) for k, g, v in con.execute(q, (counter,))]
+    import csv
+    return [(r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"]))
+            for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+
+
+def summarize_pmc(fetch_path, write_path, state_bytes, kernel="hilb_fast_kernel<7, double"):
     """HBM bytes per dispatch of the stage kernel from two rocprofv3 --pmc runs
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch).  Each counter is scaled by what
     it reads for the largest rocclr copy in the same run, a device-to-device
-    copy of one state buffer (state_bytes read, state_bytes written)."""
-    import csv
+    copy of one state buffer (state_bytes read, state_bytes written).  Dispatches
+    that leave at once (a finished Dormand-Prince interval) are left out: they
+    move less than 1% of a state buffer."""
     import statistics
 
     def load(path, counter):
-        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
-        copies = [r for r in rows if "copyBuffer" in r["Kernel_Name"]]
-        big = max(int(r["Grid_Size"]) for r in copies)
-        cal = statistics.median(float(r["Counter_Value"]) for r in copies
-                                if int(r["Grid_Size"]) == big)
-        stage = [float(r["Counter_Value"]) for r in rows if "hilb_fast_kernel" in r["Kernel_Name"]]
-        return statistics.mean(stage), cal, len(stage)
+        rows = _pmc_rows(path, counter)
+        copies = [r for r in rows if "copyBuffer" in r[0]]
+        big = max(r[1] for r in copies)
+        cal = statistics.median(r[2] for r in copies if r[1] == big)
+        stage = [r[2] for r in rows if kernel in r[0]]
+        floor = 0.01 * state_bytes / 1024.0
+        live = [v for v in stage if v > floor]
+        return statistics.mean(live), cal, len(live), len(stage) - len(live)
 
-    f, fcal, n1 = load(fetch_csv, "FETCH_SIZE")
-    w, wcal, n2 = load(write_csv, "WRITE_SIZE")
+    f, fcal, n1, e1 = load(fetch_path, "FETCH_SIZE")
+    w, wcal, n2, e2 = load(write_path, "WRITE_SIZE")
     fs, ws = state_bytes / (fcal * 1024.0), state_bytes / (wcal * 1024.0)
-    return {"fetch_bytes": f * 1024.0 * fs, "write_bytes": w * 1024.0 * ws,
+    return {"kernel": kernel, "fetch_bytes": f * 1024.0 * fs, "write_bytes": w * 1024.0 * ws,
             "traffic": f * 1024.0 * fs + w * 1024.0 * ws,
             "fetch_scale": fs, "write_scale": ws, "dispatches": [n1, n2],
+            "early_exit_dispatches_left_out": [e1, e2],
             "fetch_kib_raw": f, "write_kib_raw": w}
 
 
@@ -320,7 +337,28 @@ def bench_dense64(H, L, reps):
                     rho_s_max_abs_err_vs_fp64=float(np.abs(r["rs"] - f64["rs"]).max()),
                     speedup_time_to_rho_s=f64["time_to_rho_s_ms"] / r["time_to_rho_s_ms"])
 
+    # the same hierarchy with dense V: every bath couples through a random real
+    # symmetric V (unit spectral width); per ADO-stage 36 real 64^3 products on the
+    # matrix cores (4 for i[H, rho], 8 per bath for [V, rho], V X and Y V)
+    md = configs.dense64()
+    rng = np.random.default_rng(65)
+    for z in md["noises"]:
+        X = rng.normal(size=(64, 64))
+        Vs = 0.5 * (X + X.T)
+        w = np.linalg.eigvalsh(Vs)
+        z["V"] = (Vs / (w[-1] - w[0])).astype(np.complex128)
+    dense_v = {}
+    for dtype in ("complex128", "complex64"):
+        q = H.heom_hilb(md["max_depth"], dtype=dtype).load(md, eng)
+        sv = mk_solver(H, eng, q, H.rkdp())
+        sv.solve_rho_s(md["rho0"], t_list[:2], kw)  # builds the stages
+        st_ms, st_b = _stage_stats(L, sv, reps)
+        fl = q.n_hrchy * 36 * 2 * 64 ** 3
+        dense_v[dtype] = {"stage_ms": st_ms, "hbm_frac": st_b / (st_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "mfma_tflops": fl / (st_ms * 1e-3) / 1e12}
     return {
+        "dense_v": dict(dense_v, workload="N=64, 969 ADOs, 4 baths with dense real symmetric V, "
+                                          "one Dormand-Prince stage pass (hilb_mfma_dense_kernel)"),
         "workload": "N=64 GOE H, 4 diagonal-projector baths x (1 Drude + 3 Pade), depth 3, "
                     "969 ADOs, Dormand-Prince rtol=1e-6 atol=1e-9, t=0..20",
         "kernel": "hilb_mfma_kernel<64> (f64 / f32 16x16x4 or bf16 16x16x32 MFMA commutator, "
@@ -460,9 +498,9 @@ def main():
                     help="counter runs: set up one configuration, run one short solve and "
                          "--reps timed passes of every stage kernel, print the stage times "
                          "and exit")
-    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH", "WRITE"),
                     help="write the HBM traffic of the stage kernel from two rocprofv3 "
-                         "--pmc counter_collection.csv files and exit")
+                         "--pmc runs (their *_results.db or counter_collection.csv) and exit")
     args = ap.parse_args()
     global GRAPHS
     GRAPHS = not args.no_graphs

@@ -202,16 +202,25 @@ struct FormDev {
   const long long* R_ptr;
   const int* R_col;
   const z128* R_val;
-  // matrix-core path (heom_hilb, 16 < N <= 64, diagonal V): H as separate
-  // real / imaginary planes in FP64 and FP32, leading dimension 32 or 64 (the
-  // MFMA tile, zero-padded); mfma = 0 off, 1 on (operands in
-  // the state precision), 2 split-bf16x3 products with FP32 accumulation
+  // matrix-core path (heom_hilb, 16 < N <= 64): H as separate real / imaginary
+  // planes in FP64 and FP32, leading dimension 32 or 64 (the MFMA tile,
+  // zero-padded); mfma = 0 off, 1 on for diagonal V (operands in the state
+  // precision), 2 split-bf16x3 products with FP32 accumulation (diagonal V),
+  // 3 dense V (operands in the state precision)
   int real_h;
   int mfma;
   const double* Hre;
   const double* Him;
   const float* Hre32;
   const float* Him32;
+  // mfma == 3 (dense V, 16 < N <= 64): every V_u as real / imaginary planes
+  // of the same zero-padded tile (n_noise planes each), FP64 and FP32;
+  // real_vd: every V_u real
+  const double* Vre;
+  const double* Vim;
+  const float* Vre32;
+  const float* Vim32;
+  int real_vd;
   // learned truncation closure: rows clos_lo .. clos_lo + clos_n - 1 of the
   // ADOs (the last tier) get clos[(a - clos_lo) * 2 N^2 + e] (real part) and
   // [... + N^2 + e] (imaginary part) added to D; null when no closure is bound

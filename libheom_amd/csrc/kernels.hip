@@ -1059,6 +1059,260 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
 }
 
 // ---------------------------------------------------------------------------
+// heom_hilb on matrix cores with dense (non-diagonal) V, 16 < N <= 64
+// (heom_hilb.h:108-209): per ADO
+//   D = i[H, rho] + Gamma_n rho + sum_u (V_u X_u + Y_u V_u) + sum_direct dir rho_l
+//   X_u = i Phi_u + Psi_u + delta_u [V_u, rho],  Y_u = Psi_u - i Phi_u - delta_u [V_u, rho]
+// (i[V,Phi] + {V,Psi} + delta [V,[V,rho]] = V X + Y V).  D stays in the MFMA
+// accumulators from i[H, rho] + Gamma rho on; per noise [V_u, rho] goes to a
+// second accumulator set, X_u and then Y_u are built in an LDS workspace
+// (delta C at the accumulator positions, then the links of noise u added by
+// coalesced element-parallel passes) and multiplied on the matrix cores.
+// ---------------------------------------------------------------------------
+// acc += sgn * (A B) for the wave's tiles.  A and B are complex; an operand in
+// LDS has leading dimension LD, a global one (a V_u plane pair) N, and a global
+// operand is real when GREAL (its imaginary plane is not read).
+template <int N, typename R, bool A_LDS, bool B_LDS, bool GREAL>
+__device__ __forceinline__ void mfma_cprod(const R* ar, const R* ai, const R* br, const R* bi,
+                                           R sgn, int band, int t0, int lr, int lk,
+                                           typename Mfma<R>::acc (&cre)[MfmaCfg<N>::TPW],
+                                           typename Mfma<R>::acc (&cim)[MfmaCfg<N>::TPW]) {
+  using M = Mfma<R>;
+  using Cfg = MfmaCfg<N>;
+  constexpr int LDA = A_LDS ? Cfg::LD : N, LDB = B_LDS ? Cfg::LD : N;
+  constexpr bool AR = !A_LDS && GREAL, BR = !B_LDS && GREAL;
+  const int row = band * 16 + lr;
+#pragma unroll 2
+  for (int ks = 0; ks < Cfg::KS; ++ks) {
+    const int kk = 4 * ks + lk;
+    const R a_r = sgn * ar[row * LDA + kk];
+    const R a_i = AR ? R(0) : sgn * ai[row * LDA + kk];
+#pragma unroll
+    for (int t = 0; t < Cfg::TPW; ++t) {
+      const int col = ((t0 + t) % Cfg::NB) * 16 + lr;
+      const R b_r = br[kk * LDB + col];
+      cre[t] = M::mac(a_r, b_r, cre[t]);
+      if (!BR) {
+        const R b_i = bi[kk * LDB + col];
+        cim[t] = M::mac(a_r, b_i, cim[t]);
+        if (!AR) cre[t] = M::mac(-a_i, b_i, cre[t]);
+      }
+      if (!AR) cim[t] = M::mac(a_i, b_r, cim[t]);
+    }
+  }
+}
+
+template <typename R>
+struct MfmaV;
+template <>
+struct MfmaV<double> {
+  static __device__ __forceinline__ const double* re(const FormDev& d) { return d.Vre; }
+  static __device__ __forceinline__ const double* im(const FormDev& d) { return d.Vim; }
+};
+template <>
+struct MfmaV<float> {
+  static __device__ __forceinline__ const float* re(const FormDev& d) { return d.Vre32; }
+  static __device__ __forceinline__ const float* im(const FormDev& d) { return d.Vim32; }
+};
+
+constexpr int kDenseCh = 4;  // elements per thread whose loads are in flight together
+
+template <int N, typename R, bool REALV, bool REALH, bool F64, int KP, bool PADDED>
+__global__ void __launch_bounds__(kMfmaBS, 1)
+    hilb_mfma_dense_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  using M = Mfma<R>;
+  using Cfg = MfmaCfg<N>;
+  using acc_t = typename M::acc;
+  constexpr int N2 = Cfg::N2, LD = Cfg::LD, TPW = Cfg::TPW, NB = Cfg::NB;
+  constexpr int EPT = N2 / kMfmaBS;  // elements per thread (tile)
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  R* s_re = reinterpret_cast<R*>(s_dyn);  // rho_n, at the end D
+  R* s_im = s_re + N * LD;
+  R* w_re = s_im + N * LD;  // workspace: X_u, then Y_u
+  R* w_im = w_re + N * LD;
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int n = PADDED ? d.N : N, n2 = PADDED ? d.N2 : N * N;
+  if (PADDED) {  // the padding stays zero: only rows and columns < n are ever written
+    for (int x = threadIdx.x; x < 4 * N * LD; x += kMfmaBS) s_re[x] = R(0);
+    __syncthreads();
+  }
+  const int cur = pg.cur;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  const auto link_ptr = cst(d.link_ptr);
+  const auto flat = cst(d.flat);
+  const auto gam = cst(d.gam);
+  const auto sdl = cst(d.sdelta);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int t0 = wave * TPW;
+  const int band = t0 / NB;
+  const R* Vr0 = MfmaV<R>::re(d);
+  const R* Vi0 = MfmaV<R>::im(d);
+  double err_acc = 0.0;
+  for (int w = blockIdx.x; w < d.n_hrchy; w += gridDim.x) {
+    BlockMap bm{w, 0, 0, 0};
+    if (d.bmap) {
+      const auto p = cst(d.bmap) + w;
+      bm = BlockMap{p->g, p->l0, p->l1, 0};
+    } else {
+      bm.l0 = link_ptr[w];
+      bm.l1 = link_ptr[w + 1];
+    }
+    const int g = bm.g;
+    const long long base = static_cast<long long>(g) * n2;
+#pragma unroll 1
+    for (int c0 = 0; c0 < EPT; c0 += kDenseCh) {
+      C v[kDenseCh];
+#pragma unroll
+      for (int c = 0; c < kDenseCh; ++c) {
+        const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+        v[c] = x < n2 ? in[base + x] : C{0, 0};
+      }
+#pragma unroll
+      for (int c = 0; c < kDenseCh; ++c) {
+        const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+        if (x < n2) {
+          const int i = x / n, j = x - (x / n) * n;
+          s_re[i * LD + j] = v[c].re;
+          s_im[i * LD + j] = v[c].im;
+        }
+      }
+    }
+    __syncthreads();
+    acc_t cre[TPW], cim[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      cre[t] = acc_t{0, 0, 0, 0};
+      cim[t] = acc_t{0, 0, 0, 0};
+    }
+    mfma_commutator<N, R, REALH>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
+    // D = i[H, rho] + Gamma_n rho at the accumulator positions
+    const C gm = conv<R>(z128{gam[g].re, gam[g].im});
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int col = ((t0 + t) % NB) * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = band * 16 + M::row(lk, r);
+        const C rv{s_re[i * LD + col], s_im[i * LD + col]};
+        C Dv = times_i(C{cre[t][r], cim[t][r]});
+        cmac(Dv, gm, rv);
+        cre[t][r] = Dv.re;
+        cim[t][r] = Dv.im;
+      }
+    }
+    const int nn = d.n_noise;
+    int l = bm.l0;
+    for (int u = 0; u < nn; ++u) {
+      const R* Vr = Vr0 + static_cast<size_t>(u) * N2;
+      const R* Vi = Vi0 + static_cast<size_t>(u) * N2;
+      acc_t ccr[TPW], cci[TPW];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        ccr[t] = acc_t{0, 0, 0, 0};
+        cci[t] = acc_t{0, 0, 0, 0};
+      }
+      // C = V rho - rho V
+      mfma_cprod<N, R, false, true, REALV>(Vr, Vi, s_re, s_im, R(1), band, t0, lr, lk, ccr, cci);
+      mfma_cprod<N, R, true, false, REALV>(s_re, s_im, Vr, Vi, R(-1), band, t0, lr, lk, ccr,
+                                           cci);
+      const C dl = conv<R>(z128{sdl[u].re, sdl[u].im});
+      // links of noise u: [lu0, lu1) (the links of an ADO are sorted by noise)
+      const int lu0 = l;
+      while (l < bm.l1 && flat[l].u == u) ++l;
+      const int lu1 = l;
+#pragma unroll 1
+      for (int pass = 0; pass < 2; ++pass) {
+        // workspace = +-delta C at the accumulator positions
+        const R sg = pass == 0 ? R(1) : R(-1);
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          const int col = ((t0 + t) % NB) * 16 + lr;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = band * 16 + M::row(lk, r);
+            if (i < n && col < n) {
+              const C cv{sg * ccr[t][r], sg * cci[t][r]};
+              const C x = dl * cv;
+              w_re[i * LD + col] = x.re;
+              w_im[i * LD + col] = x.im;
+            }
+          }
+        }
+        __syncthreads();
+        // + sum_l coefficient * rho_l, element-parallel: X: iphi + psi, Y: psi - iphi
+        for (int lq = lu0; lq < lu1; ++lq) {
+          const LinkFlat q = ld_flat(flat + lq);
+          const C ip = conv<R>(q.iphi), ps = conv<R>(q.psi);
+          const C cf = pass == 0 ? ip + ps : ps - ip;
+          const C* nb = in + static_cast<long long>(q.nbr) * n2;
+#pragma unroll 1
+          for (int c0 = 0; c0 < EPT; c0 += kDenseCh) {
+            C xv[kDenseCh];
+#pragma unroll
+            for (int c = 0; c < kDenseCh; ++c) {
+              const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+              xv[c] = nb[x < n2 ? x : 0];
+            }
+#pragma unroll
+            for (int c = 0; c < kDenseCh; ++c) {
+              const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+              if (x < n2) {
+                const int i = x / n, j = x - (x / n) * n;
+                C wv{w_re[i * LD + j], w_im[i * LD + j]};
+                cmac(wv, cf, xv[c]);
+                w_re[i * LD + j] = wv.re;
+                w_im[i * LD + j] = wv.im;
+              }
+            }
+          }
+        }
+        __syncthreads();
+        if (pass == 0)  // D += V X
+          mfma_cprod<N, R, false, true, REALV>(Vr, Vi, w_re, w_im, R(1), band, t0, lr, lk, cre,
+                                               cim);
+        else  // D += Y V
+          mfma_cprod<N, R, true, false, REALV>(w_re, w_im, Vr, Vi, R(1), band, t0, lr, lk, cre,
+                                               cim);
+        __syncthreads();
+      }
+    }
+    // D to LDS (every wave has finished reading rho and the workspace)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int col = ((t0 + t) % NB) * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = band * 16 + M::row(lk, r);
+        if (i < n && col < n) {
+          s_re[i * LD + col] = cre[t][r];
+          s_im[i * LD + col] = cim[t][r];
+        }
+      }
+    }
+    __syncthreads();
+    // direct (off-diagonal gamma) links and the Runge-Kutta epilogue, coalesced
+    const int ld0 = l, ld1 = bm.l1;
+    for (int x0 = threadIdx.x; x0 < n2; x0 += kMfmaBS) {
+      const int i = x0 / n, j = x0 - (x0 / n) * n;
+      C Dc{s_re[i * LD + j], s_im[i * LD + j]};
+      PreT<KP> pv;
+      prefetch<F64, KP>(pv, A, cur, base + x0, true);
+      for (int lq = ld0; lq < ld1; ++lq) {
+        const LinkFlat q = ld_flat(flat + lq);
+        cmac(Dc, conv<R>(q.dir), in[static_cast<long long>(q.nbr) * n2 + x0]);
+      }
+      add_closure(d, base + x0, Dc);
+      err_acc += epilogue<F64, R, KP>(pv, A, cur, base + x0, Dc, pg.alpha);
+    }
+    __syncthreads();
+  }
+  if (A.do_err) write_partial(A, err_acc);
+}
+
+// ---------------------------------------------------------------------------
 // batch of Hilbert-form hierarchies (diagonal V, N <= 8): one persistent
 // launch advances every member; a 64-lane wavefront takes one work item
 // (64/N^2 ADOs of one member)
@@ -2001,8 +2255,45 @@ int launch_hilb_mfma(const FormDev& d, const StageArgs& A, hipStream_t st) {
                    : launch_hilb_mfma_n<64, R, F64, false>(d, A, st);
 }
 
+template <int N, typename R, bool F64>
+int launch_hilb_mfma_dense_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  // rho_n planes and the workspace planes
+  const size_t lds = 4 * sizeof(R) * N * MfmaCfg<N>::LD;
+  if (lds > kMfmaLdsCap)
+    throw HeomError{HEOM_ERR_UNSUPPORTED, "matrix-core path: the blocks exceed LDS"};
+  const int grid = cap_grid(d.n_hrchy);
+  const bool small = A.npre <= 4;
+#define HAMD_MD(REALV, REALH, KP, PAD)                                                 \
+  {                                                                                    \
+    auto k = hilb_mfma_dense_kernel<N, R, REALV, REALH, F64, KP, PAD>;                 \
+    static bool once = (allow_lds(k, kMfmaLdsCap), true);                              \
+    (void)once;                                                                        \
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kMfmaBS), lds, st, d, A);                   \
+  }
+#define HAMD_MD_P(REALV, REALH, KP) \
+  {                                 \
+    if (d.N == N) HAMD_MD(REALV, REALH, KP, false) else HAMD_MD(REALV, REALH, KP, true) \
+  }
+#define HAMD_MD_K(REALV, REALH) \
+  {                             \
+    if (small) HAMD_MD_P(REALV, REALH, 4) else HAMD_MD_P(REALV, REALH, kMaxPre) \
+  }
+  if (d.real_vd) {
+    if (d.real_h) HAMD_MD_K(true, true) else HAMD_MD_K(true, false)
+  } else {
+    if (d.real_h) HAMD_MD_K(false, true) else HAMD_MD_K(false, false)
+  }
+#undef HAMD_MD_K
+#undef HAMD_MD_P
+#undef HAMD_MD
+  return grid;
+}
+
 template <typename R, bool F64>
 int launch_hilb_big(const FormDev& d, const StageArgs& A, hipStream_t st) {
+  if (d.mfma == 3)  // 16 < N <= 64, dense V
+    return d.N <= 32 ? launch_hilb_mfma_dense_n<32, R, F64>(d, A, st)
+                     : launch_hilb_mfma_dense_n<64, R, F64>(d, A, st);
   if (d.mfma) return launch_hilb_mfma<R, F64>(d, A, st);  // 16 < N <= 64, diagonal V
   if (!d.diag_v)
     throw HeomError{HEOM_ERR_UNSUPPORTED,

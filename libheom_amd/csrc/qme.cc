@@ -627,8 +627,12 @@ void Qme::upload() {
   dev.R_val = d_R_val.as<z128>();
   // matrix-core commutator: heom_hilb, 16 < N <= 64, diagonal V (tiles of 32 or
   // 64 rows; H is zero-padded to the tile in its planes)
-  const bool mfma_ok = kind == HEOM_HILB && diag_v && N > 16 && N <= 64;
-  dev.mfma = (mfma_ok && opt.matrix_cores) ? (dtype == HEOM_C64_BF16X3 ? 2 : 1) : 0;
+  const bool big = kind == HEOM_HILB && N > 16 && N <= 64;
+  const bool mfma_ok = big && diag_v;
+  const bool mfma_dense = big && !diag_v;
+  dev.mfma = 0;
+  if (opt.matrix_cores && mfma_ok) dev.mfma = dtype == HEOM_C64_BF16X3 ? 2 : 1;
+  if (opt.matrix_cores && mfma_dense && dtype != HEOM_C64_BF16X3) dev.mfma = 3;
   if (dtype == HEOM_C64_BF16X3 && dev.mfma != 2)
     throw HeomError{HEOM_ERR_UNSUPPORTED,
                     "HEOM_C64_BF16X3 is provided for heom_hilb with 16 < n_level <= 64, diagonal "
@@ -637,7 +641,7 @@ void Qme::upload() {
   {
     // real / imaginary planes of H with leading dimension P (the MFMA tile,
     // zero-padded), P = N outside the matrix-core path
-    const int P = mfma_ok ? (N <= 32 ? 32 : 64) : N;
+    const int P = (mfma_ok || mfma_dense) ? (N <= 32 ? 32 : 64) : N;
     std::vector<double> hr(static_cast<size_t>(P) * P, 0.0), hi(hr.size(), 0.0);
     std::vector<float> hr32(hr.size(), 0.f), hi32(hr.size(), 0.f);
     for (int i = 0; i < N; ++i)
@@ -655,6 +659,33 @@ void Qme::upload() {
     d_Hre32.upload(hr32);
     d_Him32.upload(hi32);
   }
+  dev.real_vd = 1;
+  if (dev.mfma == 3) {
+    // every V_u zero-padded to the tile, as real / imaginary planes
+    const int P = N <= 32 ? 32 : 64;
+    const size_t pl = static_cast<size_t>(P) * P;
+    std::vector<double> vr(pl * nn, 0.0), vi(vr.size(), 0.0);
+    std::vector<float> vr32(vr.size(), 0.f), vi32(vr.size(), 0.f);
+    for (int u = 0; u < nn; ++u)
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          const z128 v = noise_work_[u].V[i * N + j];
+          const size_t e = u * pl + static_cast<size_t>(i) * P + j;
+          vr[e] = v.re;
+          vi[e] = v.im;
+          vr32[e] = static_cast<float>(v.re);
+          vi32[e] = static_cast<float>(v.im);
+          if (v.im != 0.0) dev.real_vd = 0;
+        }
+    d_Vre.upload(vr);
+    d_Vim.upload(vi);
+    d_Vre32.upload(vr32);
+    d_Vim32.upload(vi32);
+  }
+  dev.Vre = d_Vre.as<double>();
+  dev.Vim = d_Vim.as<double>();
+  dev.Vre32 = d_Vre32.as<float>();
+  dev.Vim32 = d_Vim32.as<float>();
   dev.Hre = d_Hre.as<double>();
   dev.Him = d_Him.as<double>();
   dev.Hre32 = d_Hre32.as<float>();

@@ -127,6 +127,7 @@ class Qme {
   int batch_rs_size = 0;  // sum of the members' N^2
   DevBuf d_inst, d_items;
   DevBuf d_Hre, d_Him, d_Hre32, d_Him32;
+  DevBuf d_Vre, d_Vim, d_Vre32, d_Vim32;  // dense V planes of the matrix-core path
   void autotune_batch();
 
  private:

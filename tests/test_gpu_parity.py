@@ -269,11 +269,3 @@ def test_large_n_rkdp_parity():
     rho, seen, rho_o, rs_o, st_o, st = _solve_both("hilb", m, "rkdp", kw)
     assert st["n_accepted"] == st_o.n_accepted
     assert np.abs(rho - rho_o).max() <= TOL
-
-
-def test_large_n_dense_v_is_reported_unsupported():
-    m = configs.random_model(17, 1, 1, 1, seed=2, dense_v=True)
-    g = H.heom_hilb(1).load(m, eng())
-    with pytest.raises(H.HeomError) as e:
-        g.calc_diff_impl(None, np.zeros(g.main_size(), np.complex128), m["rho0"], 1.0, 0.0)
-    assert e.value.code == 5

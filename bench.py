@@ -81,6 +81,31 @@ def summarize_pmc(fetch_path, write_path, state_bytes, kernel="hilb_fast_kernel<
             "fetch_kib_raw": f, "write_kib_raw": w}
 
 
+def summarize_trace(path, top=12, live_us=5.0):
+    """Markdown table of a rocprofv3 --kernel-trace run (its *_results.db):
+    per kernel the dispatches, total time, mean, and the mean over dispatches of
+    at least live_us (the stage passes of a finished Dormand-Prince interval
+    leave at once and take about 3 us)."""
+    import sqlite3
+    import statistics
+    con = sqlite3.connect(path)
+    rows = list(con.execute("select name, duration from kernels"))
+    by = {}
+    for name, dur in rows:
+        by.setdefault(name, []).append(dur / 1e3)  # us
+    total = sum(sum(v) for v in by.values())
+    out = ["| kernel | dispatches | total ms | %% | mean us | dispatches >= %g us | their mean us |"
+           % live_us, "|---|---|---|---|---|---|---|"]
+    for name, v in sorted(by.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        live = [x for x in v if x >= live_us]
+        short = name.replace("void ", "").replace("hamd::(anonymous namespace)::", "")
+        short = short.replace("hamd::", "").split("(")[0]
+        out.append("| `%s` | %d | %.2f | %.1f | %.2f | %d | %s |" % (
+            short, len(v), sum(v) / 1e3, 100.0 * sum(v) / total, statistics.mean(v), len(live),
+            "%.2f" % statistics.mean(live) if live else "-"))
+    return "\n".join(out)
+
+
 def _host_cpu():
     """(cores used, nproc, model name) of the host: the threads are the cores this
     process may run on, bounded by OMP_NUM_THREADS when the machine sets it (the
@@ -498,12 +523,18 @@ def main():
                     help="counter runs: set up one configuration, run one short solve and "
                          "--reps timed passes of every stage kernel, print the stage times "
                          "and exit")
+    ap.add_argument("--summarize-trace", metavar="DB",
+                    help="print a table of a rocprofv3 --kernel-trace run (*_results.db) and "
+                         "exit")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH", "WRITE"),
                     help="write the HBM traffic of the stage kernel from two rocprofv3 "
                          "--pmc runs (their *_results.db or counter_collection.csv) and exit")
     args = ap.parse_args()
     global GRAPHS
     GRAPHS = not args.no_graphs
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace))
+        return
     if args.summarize_pmc:
         from libheom_amd import configs as _c
         m0 = _c.exciton_chain7(t_final=10.0)

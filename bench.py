@@ -484,13 +484,31 @@ def bench_redfield(H, L, reps, cpu=True):
 
 def profile_stages(H, name, reps):
     """One configuration's stage kernels only (for rocprofv3 counter passes):
-    the 7-site chain (Dormand-Prince, 12650 ADOs) or the N=64 dense config
-    (Dormand-Prince, 969 ADOs), FP64, every kernel launched directly."""
+    the 7-site chain (Dormand-Prince, 12650 ADOs), the N=64 dense config
+    (Dormand-Prince, 969 ADOs), FP64, or the 512-member sweep (RK4, one batch,
+    complex64 operator then FP64), every kernel launched directly."""
     from libheom_amd import _capi, configs
     L = _capi.load()
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    m = configs.exciton_chain7(t_final=20.0) if name == "7site" else configs.dense64(t_final=0.2)
     eng = H.hip(0)
+    if name == "sweep512":
+        ms = configs.batched_sweep()
+        for dtype in ("complex64", "complex128"):
+            b = H.heom_batch([H.heom_hilb(m["max_depth"], dtype=dtype).load(m, eng) for m in ms])
+            sv = H.qme_solver(eng, b, H.rk4())
+            sv.set_graphs(False)
+            _, _ = sv.solve_rho_s(b.pack([m["rho0"] for m in ms]), ms[0]["t_list"][:2],
+                                  {"dt": ms[0]["dt"]})
+            ns = ctypes.c_int()
+            sms, sby = np.zeros(8), np.zeros(8)
+            _capi.check(L.heom_solver_time_stages(sv._s, reps, p(sms), ctypes.byref(ns)))
+            _capi.check(L.heom_solver_stage_bytes(sv._s, p(sby), ctypes.byref(ns)))
+            n = ns.value
+            print(json.dumps({"config": name, "dtype": dtype, "n_hrchy": b.n_hrchy,
+                              "stage_us": [1e3 * x for x in sms[:n]],
+                              "stage_bytes": [float(x) for x in sby[:n]]}))
+        return
+    m = configs.exciton_chain7(t_final=20.0) if name == "7site" else configs.dense64(t_final=0.2)
     q = H.heom_hilb(m["max_depth"]).load(m, eng)
     sv = H.qme_solver(eng, q, H.rkdp())
     sv.set_graphs(False)
@@ -519,7 +537,7 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--t-final", type=float, default=1000.0,
                     help="propagation length in fs (profiling runs use a shorter one)")
-    ap.add_argument("--profile-config", choices=["7site", "dense64"],
+    ap.add_argument("--profile-config", choices=["7site", "dense64", "sweep512"],
                     help="counter runs: set up one configuration, run one short solve and "
                          "--reps timed passes of every stage kernel, print the stage times "
                          "and exit")
@@ -527,8 +545,10 @@ def main():
                     help="print a table of a rocprofv3 --kernel-trace run (*_results.db) and "
                          "exit")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH", "WRITE"),
-                    help="write the HBM traffic of the stage kernel from two rocprofv3 "
-                         "--pmc runs (their *_results.db or counter_collection.csv) and exit")
+                    help="HBM traffic of the stage kernel from two rocprofv3 --pmc runs "
+                         "(their *_results.db or counter_collection.csv) of --profile-config "
+                         "--pmc-config; for 7site also written to " + TRAFFIC_JSON)
+    ap.add_argument("--pmc-config", choices=["7site", "dense64", "sweep512"], default="7site")
     args = ap.parse_args()
     global GRAPHS
     GRAPHS = not args.no_graphs
@@ -537,9 +557,18 @@ def main():
         return
     if args.summarize_pmc:
         from libheom_amd import configs as _c
-        m0 = _c.exciton_chain7(t_final=10.0)
-        t = summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], m0["rho0"].nbytes)
-        json.dump(t, open(TRAFFIC_JSON, "w"), indent=1)
+        f, w = args.summarize_pmc
+        if args.pmc_config == "7site":
+            m0 = _c.exciton_chain7(t_final=10.0)
+            t = summarize_pmc(f, w, m0["rho0"].nbytes)
+            json.dump(t, open(TRAFFIC_JSON, "w"), indent=1)
+        elif args.pmc_config == "dense64":
+            m0 = _c.dense64(t_final=0.2)
+            t = summarize_pmc(f, w, m0["rho0"].nbytes, kernel="hilb_mfma_kernel<64, double")
+        else:
+            nb = sum(m["rho0"].nbytes for m in _c.batched_sweep())
+            t = {k: summarize_pmc(f, w, nb, kernel="batch_kernel<%s" % k)
+                 for k in ("float", "double")}
         print(json.dumps(t))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

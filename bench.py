@@ -233,7 +233,10 @@ def bench_sweep(H, L, steps, warmup, reps):
         final = np.zeros_like(rho0)
         _capi.check(L.heom_memcpy_d2h(p(final), d, nb))
         L.heom_dev_free(d)
+        chosen, tried = b.tuning()
         res[dtype] = {"value": st["ado_stage_updates"] / el, "ms_per_step": 1e3 * el / steps,
+                      "autotune": dict(chosen, tried_us_per_pass={"%dx%d" % (k, g): round(u, 1)
+                                                                  for k, g, u in tried}),
                       "stage_ms": float(sms[:n].mean()), "stage_bytes": float(sby[:n].mean()),
                       "hbm_frac": gbs / HBM_PEAK_GBS, "rs": b.split_rho_s(rs),
                       "final": b.unpack(final),
@@ -272,9 +275,10 @@ def bench_sweep(H, L, steps, warmup, reps):
         "n_hrchy_total": f64["n_hrchy"], "main_size": f64["main_size"],
         "dtype": "f32 operator / f64 combination", "value": f32["value"],
         "ms_per_step": f32["ms_per_step"], "stage_ms": f32["stage_ms"],
-        "hbm_frac": f32["hbm_frac"],
+        "hbm_frac": f32["hbm_frac"], "autotune": f32["autotune"],
         "fp64": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
-                 "stage_ms": f64["stage_ms"], "hbm_frac": f64["hbm_frac"]},
+                 "stage_ms": f64["stage_ms"], "hbm_frac": f64["hbm_frac"],
+                 "autotune": f64["autotune"]},
         "fp32_batch_speedup_vs_fp64_batch": f64["ms_per_step"] / f32["ms_per_step"],
         "rho_s_max_abs_err_vs_fp64_bounded_members": worst,
         "bounded_members": len(ms) - n_div,

@@ -162,6 +162,17 @@ int heom_batch_create(const heom_handle* members, int n_members, heom_handle* ou
  * n_level (any array may be NULL) */
 int heom_batch_layout(heom_handle batch, long long* elem_base, int* rho_s_offset,
                       int* n_level);
+/* The batch kernel's autotuner (no counterpart in the reference): at
+ * heom_batch_create it times one derivative pass of the whole batch for every
+ * tile shape (work items per wavefront step, 1 or 2; a work item is 64/n_level^2
+ * ADOs of one member) and grid size, and keeps the fastest.  Writes the chosen
+ * shape and grid, and up to `cap` timed candidates (items per step, grid, mean
+ * microseconds per pass) with their count in *n_tried. */
+int heom_batch_tuning(heom_handle batch, int* items_per_step, int* grid, int* tried_ipw,
+                      int* tried_grid, float* tried_us, int cap, int* n_tried);
+/* Overrides the autotuner's choice (items_per_step 1 or 2, 1 <= grid <= 262144); the
+ * results do not depend on the shape, only the speed does. */
+int heom_batch_set_tuning(heom_handle batch, int items_per_step, int grid);
 
 /* Learned truncation closure (no counterpart in the reference, which drops
  * every ADO beyond max_depth; heom.h:77-91).  A 3-layer network (n_in ->

@@ -29,7 +29,7 @@ EXPORTS = [
     "heom_last_error", "heom_version", "heom_default_options", "heom_create", "heom_destroy",
     "heom_set_H", "heom_alloc_noises", "heom_set_noise", "heom_set_hrchy_filter", "heom_set_param", "heom_get_n_hrchy",
     "heom_get_n_dim", "heom_main_size", "heom_get_hrchy", "heom_redfield_lambda", "heom_calc_diff",
-    "heom_calc_diff_host", "heom_batch_create", "heom_batch_layout", "heom_solver_create", "heom_solver_destroy", "heom_solve",
+    "heom_calc_diff_host", "heom_batch_create", "heom_batch_layout", "heom_batch_tuning", "heom_batch_set_tuning", "heom_solver_create", "heom_solver_destroy", "heom_solve",
     "heom_solve_dev", "heom_solve_dev_noalloc", "heom_solver_stats", "heom_solver_reset", "heom_solver_set_graphs", "heom_solver_time_stages",
     "heom_solver_stage_bytes", "heom_dev_alloc", "heom_dev_free", "heom_memcpy_h2d",
     "heom_memcpy_d2h", "heom_device_sync", "heom_stream_copy", "heom_mlp_create", "heom_mlp_destroy",
@@ -102,6 +102,8 @@ def load():
         "heom_calc_diff_host": (ip, [vp, vp, vp, dp, dp, dp, dp]),
         "heom_batch_create": (ip, [vp, ip, P(vp)]),
         "heom_batch_layout": (ip, [vp, vp, vp, vp]),
+        "heom_batch_tuning": (ip, [vp, vp, vp, vp, vp, vp, ip, vp]),
+        "heom_batch_set_tuning": (ip, [vp, ip, ip]),
         "heom_solver_create": (ip, [vp, ip, P(vp)]),
         "heom_solver_destroy": (ip, [vp]),
         "heom_solve": (ip, [vp, vp, vp, ip, CALLBACK, vp, dp, dp, dp, ip]),

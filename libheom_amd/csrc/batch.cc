@@ -101,12 +101,16 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
   dev.inst = d_inst.as<BatchInst>();
   dev.items = d_items.as<BatchItem>();
   dev.batch_grid = std::min(dev.n_items, kMaxGrid);
+  dev.batch_ipw = 1;
   ready = true;
   autotune_batch();
 }
 
-// Grid-size autotuner of the persistent batch kernel: time one derivative
-// evaluation of the whole batch for a few grid sizes and keep the fastest.
+// Autotuner of the persistent batch kernel: time one derivative evaluation of
+// the whole batch for every tile shape (work items per wavefront step, 1 or 2)
+// and grid size (512 .. 16384 workgroups, or one per item group), keep the
+// fastest.  The results are the same for every choice (each element's
+// arithmetic does not depend on it).
 void Qme::autotune_batch() {
   const long long ms = main_size();
   const size_t eb = lo() ? sizeof(z64) : sizeof(z128);
@@ -123,34 +127,44 @@ void Qme::autotune_batch() {
   A.dst_lo[0] = A.in_lo;
   A.a_re = 1.0;
   const int rb = lo() ? 4 : 8;
-  std::vector<int> cands;
-  for (int g : {512, 1024, 2048, 4096, 8192, 16384})
-    if (g < dev.n_items) cands.push_back(g);
-  cands.push_back(std::min(dev.n_items, kMaxGrid));
   hipEvent_t e0, e1;
   HAMD_HIP(hipEventCreate(&e0));
   HAMD_HIP(hipEventCreate(&e1));
   float best = std::numeric_limits<float>::max();
-  int best_g = cands.back();
-  for (int g : cands) {
-    dev.batch_grid = g;
-    launch_operator(dev, rb, A, stream);  // warm-up
-    for (int round = 0; round < 3; ++round) {  // best of three rounds of five passes
-      HAMD_HIP(hipEventRecord(e0, stream));
-      for (int r = 0; r < 5; ++r) launch_operator(dev, rb, A, stream);
-      HAMD_HIP(hipEventRecord(e1, stream));
-      HAMD_HIP(hipEventSynchronize(e1));
-      float ms_ = 0.f;
-      HAMD_HIP(hipEventElapsedTime(&ms_, e0, e1));
-      if (ms_ < best) {
-        best = ms_;
+  int best_g = std::min(dev.n_items, kMaxGrid), best_ipw = 1;
+  tune_log.clear();
+  for (int ipw : {1, 2}) {
+    const int groups = (dev.n_items + ipw - 1) / ipw;
+    std::vector<int> cands;
+    for (int g : {512, 1024, 2048, 4096, 8192, 16384})
+      if (g < groups) cands.push_back(g);
+    cands.push_back(std::min(groups, kMaxGrid));
+    for (int g : cands) {
+      dev.batch_grid = g;
+      dev.batch_ipw = ipw;
+      launch_operator(dev, rb, A, stream);  // warm-up
+      float shape_best = std::numeric_limits<float>::max();
+      for (int round = 0; round < 3; ++round) {  // best of three rounds of five passes
+        HAMD_HIP(hipEventRecord(e0, stream));
+        for (int r = 0; r < 5; ++r) launch_operator(dev, rb, A, stream);
+        HAMD_HIP(hipEventRecord(e1, stream));
+        HAMD_HIP(hipEventSynchronize(e1));
+        float ms_ = 0.f;
+        HAMD_HIP(hipEventElapsedTime(&ms_, e0, e1));
+        shape_best = std::min(shape_best, ms_);
+      }
+      tune_log.push_back(TuneEntry{ipw, g, 1e3f * shape_best / 5.0f});
+      if (shape_best < best) {
+        best = shape_best;
         best_g = g;
+        best_ipw = ipw;
       }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   dev.batch_grid = best_g;
+  dev.batch_ipw = best_ipw;
 }
 
 }  // namespace hamd

@@ -331,6 +331,37 @@ int heom_batch_layout(heom_handle batch, long long* elem_base, int* rho_s_offset
   });
 }
 
+int heom_batch_tuning(heom_handle batch, int* items_per_step, int* grid, int* tried_ipw,
+                      int* tried_grid, float* tried_us, int cap, int* n_tried) {
+  return guard([&] {
+    hamd::Qme& q = Qready(batch);
+    if (q.kind != hamd::kBatch) throw hamd::HeomError{HEOM_ERR_ARG, "not a batch handle"};
+    if (items_per_step) *items_per_step = q.dev.batch_ipw;
+    if (grid) *grid = q.dev.batch_grid;
+    const int n = static_cast<int>(q.tune_log.size());
+    for (int k = 0; k < n && k < cap; ++k) {
+      if (tried_ipw) tried_ipw[k] = q.tune_log[k].ipw;
+      if (tried_grid) tried_grid[k] = q.tune_log[k].grid;
+      if (tried_us) tried_us[k] = q.tune_log[k].us;
+    }
+    if (n_tried) *n_tried = n;
+    return HEOM_OK;
+  });
+}
+
+int heom_batch_set_tuning(heom_handle batch, int items_per_step, int grid) {
+  return guard([&] {
+    hamd::Qme& q = Qready(batch);
+    if (q.kind != hamd::kBatch) throw hamd::HeomError{HEOM_ERR_ARG, "not a batch handle"};
+    if ((items_per_step != 1 && items_per_step != 2) || grid < 1 || grid > hamd::kMaxGrid)
+      throw hamd::HeomError{HEOM_ERR_ARG, "heom_batch_set_tuning: items_per_step must be 1 or 2 "
+                                          "and grid in 1 .. 262144"};
+    q.dev.batch_ipw = items_per_step;
+    q.dev.batch_grid = grid;
+    return HEOM_OK;
+  });
+}
+
 int heom_mlp_create(int n_in, int n_hidden, int n_out, unsigned long long seed, int device,
                     heom_mlp_handle* out) {
   return guard([&] {

@@ -236,6 +236,7 @@ struct FormDev {
   const z128* rf_HR;
   // batch
   int n_inst, n_items, batch_grid;
+  int batch_ipw;  // work items per wavefront step (1 or 2), picked by the autotuner
   const BatchInst* inst;
   const BatchItem* items;
 };

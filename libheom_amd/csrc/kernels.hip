@@ -771,23 +771,25 @@ struct MfmaCfg {
 };
 
 // H*rho - rho*H for the wave's tiles, operands in the state precision R
-template <int N, typename R, bool REALH>
+template <int N, typename R, bool REALH, int TPW>
 __device__ __forceinline__ void mfma_commutator(const FormDev& d, const R* s_re, const R* s_im,
                                                 int band, int t0, int lr, int lk,
-                                                typename Mfma<R>::acc (&cre)[MfmaCfg<N>::TPW],
-                                                typename Mfma<R>::acc (&cim)[MfmaCfg<N>::TPW]) {
+                                                typename Mfma<R>::acc (&cre)[TPW],
+                                                typename Mfma<R>::acc (&cim)[TPW]) {
   using M = Mfma<R>;
   using Cfg = MfmaCfg<N>;
   const R* Hr = M::re(d);
   const R* Hi = M::im(d);
   const int row = band * 16 + lr;
-#pragma unroll
+  // fewer tiles per wave go with a smaller register budget (see hilb_mfma_kernel)
+  constexpr int KU = TPW > 2 ? Cfg::KS : 4;
+#pragma unroll KU
   for (int ks = 0; ks < Cfg::KS; ++ks) {
     const int kk = 4 * ks + lk;
     const R arr = -s_re[row * Cfg::LD + kk], ari = -s_im[row * Cfg::LD + kk];
     const R ahr = Hr[row * N + kk], ahi = REALH ? R(0) : Hi[row * N + kk];
 #pragma unroll
-    for (int t = 0; t < Cfg::TPW; ++t) {
+    for (int t = 0; t < TPW; ++t) {
       const int col = ((t0 + t) % Cfg::NB) * 16 + lr;
       const R brr = s_re[kk * Cfg::LD + col], bri = s_im[kk * Cfg::LD + col];
       const R bhr = Hr[kk * N + col];
@@ -807,12 +809,11 @@ __device__ __forceinline__ void mfma_commutator(const FormDev& d, const R* s_re,
 }
 
 // the same products on bf16 matrix cores with split operands (FP32 state)
-template <int N, bool REALH>
+template <int N, bool REALH, int TPW>
 __device__ __forceinline__ void mfma_commutator_bf16x3(const FormDev& d, const float* s_re,
                                                        const float* s_im, int band, int t0,
-                                                       int lr, int lk,
-                                                       mf_f4 (&cre)[MfmaCfg<N>::TPW],
-                                                       mf_f4 (&cim)[MfmaCfg<N>::TPW]) {
+                                                       int lr, int lk, mf_f4 (&cre)[TPW],
+                                                       mf_f4 (&cim)[TPW]) {
   using Cfg = MfmaCfg<N>;
   const float* Hr = d.Hre32;
   const float* Hi = d.Him32;
@@ -838,7 +839,7 @@ __device__ __forceinline__ void mfma_commutator_bf16x3(const FormDev& d, const f
     const Split8 ari = split8(x, true);   // -rho_i
     const Split8 pari = split8(x, false);
 #pragma unroll
-    for (int t = 0; t < Cfg::TPW; ++t) {
+    for (int t = 0; t < TPW; ++t) {
       const int col = ((t0 + t) % Cfg::NB) * 16 + lr;
 #pragma unroll
       for (int q = 0; q < 8; ++q) x[q] = s_re[(k0 + q) * Cfg::LD + col];
@@ -866,19 +867,22 @@ __device__ __forceinline__ void mfma_commutator_bf16x3(const FormDev& d, const f
   }
 }
 
-// Every form asks for two waves per SIMD (the register budget of 256 per lane;
-// LDS holds two FP64 workgroups per CU).
+// BS threads (BS / 64 waves) per ADO; every form asks for two workgroups per
+// CU (LDS holds two FP64 workgroups per CU), i.e. BS / 128 waves per SIMD:
+// 256 threads leave 256 registers per lane, 512 threads 128.
 // BF16X3: R = float, products via mfma_commutator_bf16x3.  N is the tile size
 // (32 or 64); the ADO block is n x n with n = d.N in (N/2, N] for N = 64, in
 // (16, 32] for N = 32, zero-padded in LDS (and H in its planes), so that the
 // padded rows and columns of the products are exactly zero.
-template <int N, typename R, bool REALH, bool F64, int KP, bool BF16X3, bool PADDED>
-__global__ void __launch_bounds__(kMfmaBS, 2)
+template <int N, typename R, bool REALH, bool F64, int KP, bool BF16X3, bool PADDED, int BS>
+__global__ void __launch_bounds__(BS, BS / 128)
     hilb_mfma_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   using M = Mfma<R>;
   using Cfg = MfmaCfg<N>;
-  constexpr int N2 = Cfg::N2, LD = Cfg::LD, TPW = Cfg::TPW, NB = Cfg::NB;
+  constexpr int N2 = Cfg::N2, LD = Cfg::LD, NB = Cfg::NB;
+  constexpr int TPW = NB * NB / (BS / 64);  // output tiles per wave
+  static_assert(TPW >= 1 && TPW * (BS / 64) == NB * NB, "tiles must divide among the waves");
   extern __shared__ __align__(16) unsigned char s_dyn[];
   R* s_re = reinterpret_cast<R*>(s_dyn);
   R* s_im = s_re + N * LD;
@@ -888,11 +892,11 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
   // the ADO block (n <= N); unpadded blocks keep the index arithmetic constant
   const int n = PADDED ? d.N : N, n2 = PADDED ? d.N2 : N * N;
   if (PADDED) {  // the padding stays zero: only rows and columns < n are ever written
-    for (int x = threadIdx.x; x < 2 * N * LD; x += kMfmaBS) s_re[x] = R(0);
+    for (int x = threadIdx.x; x < 2 * N * LD; x += BS) s_re[x] = R(0);
   }
   {
     const int nvd = (d.n_noise + 1) * d.N, nvd0 = d.n_noise * d.N;
-    for (int x = threadIdx.x; x < nvd; x += kMfmaBS) {
+    for (int x = threadIdx.x; x < nvd; x += BS) {
       const z128 v = x < nvd0 ? d.Vd[x] : z128{0.0, 0.0};
       if (d.real_v) {
         s_v[x] = static_cast<R>(v.re);
@@ -927,16 +931,16 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
     const long long base = static_cast<long long>(g) * n2;
     {
       // all loads of rho_n in flight before the LDS stores
-      constexpr int PER = N2 / kMfmaBS;
+      constexpr int PER = N2 / BS;
       C v[PER];
 #pragma unroll
       for (int c = 0; c < PER; ++c) {
-        const int x = threadIdx.x + c * kMfmaBS;
+        const int x = threadIdx.x + c * BS;
         v[c] = x < n2 ? in[base + x] : C{0, 0};
       }
 #pragma unroll
       for (int c = 0; c < PER; ++c) {
-        const int x = threadIdx.x + c * kMfmaBS;
+        const int x = threadIdx.x + c * BS;
         if (x < n2) {
           const int i = x / n, j = x - (x / n) * n;
           s_re[i * LD + j] = v[c].re;
@@ -952,9 +956,9 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
       cim[t] = typename M::acc{0, 0, 0, 0};
     }
     if constexpr (BF16X3)
-      mfma_commutator_bf16x3<N, REALH>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
+      mfma_commutator_bf16x3<N, REALH, TPW>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
     else
-      mfma_commutator<N, R, REALH>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
+      mfma_commutator<N, R, REALH, TPW>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
     // every wave has read rho_n as MFMA operands: each lane now replaces its
     // own elements by i[H, rho] + (Gamma_n + delta term) rho (one owner each)
     __syncthreads();
@@ -988,9 +992,10 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
     // before its stores (the stores may alias later loads); elements past the
     // block read element 0 and store nothing.
     constexpr bool kLinksOuter = sizeof(R) == 8 || BF16X3;
-    constexpr int EPT = N2 / kMfmaBS;         // elements per thread (tile)
-    constexpr int LCH = EPT < 8 ? EPT : 8;    // elements per load group
-    constexpr int CH = KP <= 4 ? 2 : 1;
+    constexpr int EPT = N2 / BS;              // elements per thread (tile)
+    constexpr int LMAX = BS > 256 ? 4 : 8;    // register budget: 256 / 128 per lane
+    constexpr int LCH = EPT < LMAX ? EPT : LMAX;  // elements per load group
+    constexpr int CH = (KP <= 4 && BS <= 256) ? 2 : 1;
     const int l0 = bm.l0, l1 = bm.l1;
     if constexpr (kLinksOuter) {
       for (int l = l0; l < l1; ++l) {
@@ -1001,13 +1006,13 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
           C xv[LCH];
 #pragma unroll
           for (int c = 0; c < LCH; ++c) {
-            const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+            const int x = threadIdx.x + (c0 + c) * BS;
             const int xs = x < n2 ? x : 0;
             xv[c] = nb[xs];
           }
 #pragma unroll
           for (int c = 0; c < LCH; ++c) {
-            const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+            const int x = threadIdx.x + (c0 + c) * BS;
             if (x < n2) {
               const int i = x / n, j = x - (x / n) * n;
               C Dv{s_re[i * LD + j], s_im[i * LD + j]};
@@ -1019,13 +1024,13 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
         }
       }
     }
-    for (int x0 = threadIdx.x; x0 < n2; x0 += kMfmaBS * CH) {
+    for (int x0 = threadIdx.x; x0 < n2; x0 += BS * CH) {
       C Dc[CH];
       PreT<KP> pv[CH];
       int xs[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
-        const int x = x0 + c * kMfmaBS;
+        const int x = x0 + c * BS;
         xs[c] = x < n2 ? x : 0;
         const int i = xs[c] / n, j = xs[c] - (xs[c] / n) * n;
         Dc[c] = C{s_re[i * LD + j], s_im[i * LD + j]};
@@ -1046,7 +1051,7 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
       }
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
-        const int x = x0 + c * kMfmaBS;
+        const int x = x0 + c * BS;
         if (x < n2) {
           add_closure(d, base + x, Dc[c]);
           err_acc += epilogue<F64, R, KP>(pv[c], A, cur, base + x, Dc[c], pg.alpha);
@@ -1072,11 +1077,11 @@ __global__ void __launch_bounds__(kMfmaBS, 2)
 // acc += sgn * (A B) for the wave's tiles.  A and B are complex; an operand in
 // LDS has leading dimension LD, a global one (a V_u plane pair) N, and a global
 // operand is real when GREAL (its imaginary plane is not read).
-template <int N, typename R, bool A_LDS, bool B_LDS, bool GREAL>
+template <int N, typename R, bool A_LDS, bool B_LDS, bool GREAL, int TPW>
 __device__ __forceinline__ void mfma_cprod(const R* ar, const R* ai, const R* br, const R* bi,
                                            R sgn, int band, int t0, int lr, int lk,
-                                           typename Mfma<R>::acc (&cre)[MfmaCfg<N>::TPW],
-                                           typename Mfma<R>::acc (&cim)[MfmaCfg<N>::TPW]) {
+                                           typename Mfma<R>::acc (&cre)[TPW],
+                                           typename Mfma<R>::acc (&cim)[TPW]) {
   using M = Mfma<R>;
   using Cfg = MfmaCfg<N>;
   constexpr int LDA = A_LDS ? Cfg::LD : N, LDB = B_LDS ? Cfg::LD : N;
@@ -1088,7 +1093,7 @@ __device__ __forceinline__ void mfma_cprod(const R* ar, const R* ai, const R* br
     const R a_r = sgn * ar[row * LDA + kk];
     const R a_i = AR ? R(0) : sgn * ai[row * LDA + kk];
 #pragma unroll
-    for (int t = 0; t < Cfg::TPW; ++t) {
+    for (int t = 0; t < TPW; ++t) {
       const int col = ((t0 + t) % Cfg::NB) * 16 + lr;
       const R b_r = br[kk * LDB + col];
       cre[t] = M::mac(a_r, b_r, cre[t]);
@@ -1117,15 +1122,16 @@ struct MfmaV<float> {
 
 constexpr int kDenseCh = 4;  // elements per thread whose loads are in flight together
 
-template <int N, typename R, bool REALV, bool REALH, bool F64, int KP, bool PADDED>
-__global__ void __launch_bounds__(kMfmaBS, 1)
+template <int N, typename R, bool REALV, bool REALH, bool F64, int KP, bool PADDED, int BS>
+__global__ void __launch_bounds__(BS, BS / 256)
     hilb_mfma_dense_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
   using M = Mfma<R>;
   using Cfg = MfmaCfg<N>;
   using acc_t = typename M::acc;
-  constexpr int N2 = Cfg::N2, LD = Cfg::LD, TPW = Cfg::TPW, NB = Cfg::NB;
-  constexpr int EPT = N2 / kMfmaBS;  // elements per thread (tile)
+  constexpr int N2 = Cfg::N2, LD = Cfg::LD, NB = Cfg::NB;
+  constexpr int TPW = NB * NB / (BS / 64);  // output tiles per wave
+  constexpr int EPT = N2 / BS;  // elements per thread (tile)
   extern __shared__ __align__(16) unsigned char s_dyn[];
   R* s_re = reinterpret_cast<R*>(s_dyn);  // rho_n, at the end D
   R* s_im = s_re + N * LD;
@@ -1135,7 +1141,7 @@ __global__ void __launch_bounds__(kMfmaBS, 1)
   if (pg.skip) return;
   const int n = PADDED ? d.N : N, n2 = PADDED ? d.N2 : N * N;
   if (PADDED) {  // the padding stays zero: only rows and columns < n are ever written
-    for (int x = threadIdx.x; x < 4 * N * LD; x += kMfmaBS) s_re[x] = R(0);
+    for (int x = threadIdx.x; x < 4 * N * LD; x += BS) s_re[x] = R(0);
     __syncthreads();
   }
   const int cur = pg.cur;
@@ -1167,12 +1173,12 @@ __global__ void __launch_bounds__(kMfmaBS, 1)
       C v[kDenseCh];
 #pragma unroll
       for (int c = 0; c < kDenseCh; ++c) {
-        const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+        const int x = threadIdx.x + (c0 + c) * BS;
         v[c] = x < n2 ? in[base + x] : C{0, 0};
       }
 #pragma unroll
       for (int c = 0; c < kDenseCh; ++c) {
-        const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+        const int x = threadIdx.x + (c0 + c) * BS;
         if (x < n2) {
           const int i = x / n, j = x - (x / n) * n;
           s_re[i * LD + j] = v[c].re;
@@ -1187,7 +1193,7 @@ __global__ void __launch_bounds__(kMfmaBS, 1)
       cre[t] = acc_t{0, 0, 0, 0};
       cim[t] = acc_t{0, 0, 0, 0};
     }
-    mfma_commutator<N, R, REALH>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
+    mfma_commutator<N, R, REALH, TPW>(d, s_re, s_im, band, t0, lr, lk, cre, cim);
     // D = i[H, rho] + Gamma_n rho at the accumulator positions
     const C gm = conv<R>(z128{gam[g].re, gam[g].im});
 #pragma unroll
@@ -1215,9 +1221,10 @@ __global__ void __launch_bounds__(kMfmaBS, 1)
         cci[t] = acc_t{0, 0, 0, 0};
       }
       // C = V rho - rho V
-      mfma_cprod<N, R, false, true, REALV>(Vr, Vi, s_re, s_im, R(1), band, t0, lr, lk, ccr, cci);
-      mfma_cprod<N, R, true, false, REALV>(s_re, s_im, Vr, Vi, R(-1), band, t0, lr, lk, ccr,
-                                           cci);
+      mfma_cprod<N, R, false, true, REALV, TPW>(Vr, Vi, s_re, s_im, R(1), band, t0, lr, lk, ccr,
+                                                cci);
+      mfma_cprod<N, R, true, false, REALV, TPW>(s_re, s_im, Vr, Vi, R(-1), band, t0, lr, lk, ccr,
+                                                cci);
       const C dl = conv<R>(z128{sdl[u].re, sdl[u].im});
       // links of noise u: [lu0, lu1) (the links of an ADO are sorted by noise)
       const int lu0 = l;
@@ -1253,12 +1260,12 @@ __global__ void __launch_bounds__(kMfmaBS, 1)
             C xv[kDenseCh];
 #pragma unroll
             for (int c = 0; c < kDenseCh; ++c) {
-              const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+              const int x = threadIdx.x + (c0 + c) * BS;
               xv[c] = nb[x < n2 ? x : 0];
             }
 #pragma unroll
             for (int c = 0; c < kDenseCh; ++c) {
-              const int x = threadIdx.x + (c0 + c) * kMfmaBS;
+              const int x = threadIdx.x + (c0 + c) * BS;
               if (x < n2) {
                 const int i = x / n, j = x - (x / n) * n;
                 C wv{w_re[i * LD + j], w_im[i * LD + j]};
@@ -1271,11 +1278,11 @@ __global__ void __launch_bounds__(kMfmaBS, 1)
         }
         __syncthreads();
         if (pass == 0)  // D += V X
-          mfma_cprod<N, R, false, true, REALV>(Vr, Vi, w_re, w_im, R(1), band, t0, lr, lk, cre,
-                                               cim);
+          mfma_cprod<N, R, false, true, REALV, TPW>(Vr, Vi, w_re, w_im, R(1), band, t0, lr, lk,
+                                                    cre, cim);
         else  // D += Y V
-          mfma_cprod<N, R, true, false, REALV>(w_re, w_im, Vr, Vi, R(1), band, t0, lr, lk, cre,
-                                               cim);
+          mfma_cprod<N, R, true, false, REALV, TPW>(w_re, w_im, Vr, Vi, R(1), band, t0, lr, lk,
+                                                    cre, cim);
         __syncthreads();
       }
     }
@@ -1295,7 +1302,7 @@ __global__ void __launch_bounds__(kMfmaBS, 1)
     __syncthreads();
     // direct (off-diagonal gamma) links and the Runge-Kutta epilogue, coalesced
     const int ld0 = l, ld1 = bm.l1;
-    for (int x0 = threadIdx.x; x0 < n2; x0 += kMfmaBS) {
+    for (int x0 = threadIdx.x; x0 < n2; x0 += BS) {
       const int i = x0 / n, j = x0 - (x0 / n) * n;
       C Dc{s_re[i * LD + j], s_im[i * LD + j]};
       PreT<KP> pv;
@@ -1314,96 +1321,124 @@ __global__ void __launch_bounds__(kMfmaBS, 1)
 
 // ---------------------------------------------------------------------------
 // batch of Hilbert-form hierarchies (diagonal V, N <= 8): one persistent
-// launch advances every member; a 64-lane wavefront takes one work item
-// (64/N^2 ADOs of one member)
+// launch advances every member; a 64-lane wavefront takes IPW consecutive
+// work items per step (each 64/N^2 ADOs of one member), loading all of them
+// before it computes any (IPW is a tile shape the batch autotuner picks)
 // ---------------------------------------------------------------------------
-// One work item of the batch kernel: apw = 64/N^2 ADOs of one member.
+// The lane's share of one work item.
+template <typename R, int KP>
+struct BatchLane {
+  BatchInst I;
+  int ado, e, i, j;
+  bool ok;
+  long long gi;
+  cx<R> r;
+  PreT<KP> pv;
+};
+
+// Load phase of work item `it`: the lane's element, its epilogue operands,
+// and the item's H and ADO blocks in LDS.
 template <typename R, bool F64, int KP>
-__device__ __forceinline__ void batch_item(const FormDev& d, const StageArgs& A, const cx<R>* in,
-                                           const BatchInst& I, int ado0, int cur, z128 alpha,
-                                           cx<R>* s_rho, cx<R>* s_H, double& err_acc) {
+__device__ __forceinline__ void batch_load(const FormDev& d, const StageArgs& A, const cx<R>* in,
+                                           int it, int cur, cx<R>* s_rho, cx<R>* s_H,
+                                           BatchLane<R, KP>& b) {
   using C = cx<R>;
   const int lane = threadIdx.x;
-  const int N = I.N, N2 = I.N2, apw = I.apw;
+  const auto items = cst(d.items);
+  const auto insts = cst(d.inst);
+  const bool live = it < d.n_items;
+  const int m = live ? items[it].inst : 0, ado0 = live ? items[it].ado : 0;
+  BatchInst& I = b.I;
+  I.N = insts[m].N;
+  I.N2 = insts[m].N2;
+  I.n_hrchy = insts[m].n_hrchy;
+  I.apw = insts[m].apw;
+  I.ado_base = insts[m].ado_base;
+  I.h_off = insts[m].h_off;
+  I.tab_off = insts[m].tab_off;
+  I.dterm_off = insts[m].dterm_off;
+  I.real_v = insts[m].real_v;
+  I.elem_base = insts[m].elem_base;
+  const int N = I.N, N2 = I.N2;
   const int la = lane / N2;
-  const int e = lane - la * N2;
-  const int i = e / N, j = e - (e / N) * N;
-  const int ado = ado0 + la;
-  const bool ok = la < apw && ado < I.n_hrchy;
-  const long long gi = I.elem_base + static_cast<long long>(ok ? ado : ado0) * N2 + (ok ? e : 0);
-  const C r = in[gi];
-  PreT<KP> pv;
-  prefetch<F64, KP>(pv, A, cur, gi, ok);
+  b.e = lane - la * N2;
+  b.i = b.e / N;
+  b.j = b.e - b.i * N;
+  b.ado = ado0 + la;
+  b.ok = live && la < I.apw && b.ado < I.n_hrchy;
+  b.gi = I.elem_base + static_cast<long long>(b.ok ? b.ado : ado0) * N2 + (b.ok ? b.e : 0);
+  b.r = in[b.gi];
+  prefetch<F64, KP>(b.pv, A, cur, b.gi, b.ok);
   s_H[lane] = lane < N2 ? conv<R>(d.H[I.h_off + lane]) : C{0, 0};
-  s_rho[lane] = ok ? r : C{0, 0};
-  __syncthreads();
+  s_rho[lane] = b.ok ? b.r : C{0, 0};
+}
+
+// Compute phase: D of the lane's element and the Runge-Kutta epilogue.
+template <typename R, bool F64, int KP>
+__device__ __forceinline__ void batch_compute(const FormDev& d, const StageArgs& A,
+                                              const cx<R>* in, int cur, z128 alpha,
+                                              const cx<R>* s_rho, const cx<R>* s_H,
+                                              const BatchLane<R, KP>& b, double& err_acc) {
+  using C = cx<R>;
+  if (!b.ok) return;
+  const BatchInst& I = b.I;
+  const int N = I.N, N2 = I.N2, i = b.i, j = b.j, e = b.e;
   C acc{0, 0};
-  if (ok) {
-    const C* b = s_rho + la * N2;
-    // members have N <= 8: unrolled to 8 with an exit (measured faster than the
-    // rolled loop in FP64)
+  const C* bl = s_rho + (threadIdx.x / N2) * N2;
+  // members have N <= 8: unrolled to 8 with an exit (measured faster than the
+  // rolled loop in FP64)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k >= N) break;
-      cmac(acc, s_H[i * N + k], b[k * N + j]);
-      const C t = b[i * N + k];
-      cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
-    }
+  for (int k = 0; k < 8; ++k) {
+    if (k >= N) break;
+    cmac(acc, s_H[i * N + k], bl[k * N + j]);
+    const C t = bl[i * N + k];
+    cmac(acc, C{-t.re, -t.im}, s_H[k * N + j]);
   }
   C D = times_i(acc);
-  if (ok) {
-    cmac(D, conv<R>(d.gam[I.ado_base + ado]) + conv<R>(d.dterm[I.dterm_off + e]), r);
-    const int l0 = d.link_ptr[I.ado_base + ado], l1 = d.link_ptr[I.ado_base + ado + 1];
-    for (int l = l0; l < l1; ++l) {
-      const LinkFlat q = d.flat[l];
-      const C x = in[I.elem_base + static_cast<long long>(q.nbr) * N2 + e];
-      const int tb = q.u * N2 + e;
-      C cf;
-      if (I.real_v) {
-        const z128 p = d.cmac_tab[I.tab_off + tb];
-        cf.re = static_cast<R>(p.re) * static_cast<R>(q.iphi.re) +
-                static_cast<R>(p.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
-        cf.im = static_cast<R>(p.re) * static_cast<R>(q.iphi.im) +
-                static_cast<R>(p.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
-      } else {
-        cf = conv<R>(d.cmac_tab[I.tab_off + 2 * tb]) * conv<R>(q.iphi) +
-             conv<R>(d.cmac_tab[I.tab_off + 2 * tb + 1]) * conv<R>(q.psi) + conv<R>(q.dir);
-      }
-      cmac(D, cf, x);
+  cmac(D, conv<R>(d.gam[I.ado_base + b.ado]) + conv<R>(d.dterm[I.dterm_off + e]), b.r);
+  const int l0 = d.link_ptr[I.ado_base + b.ado], l1 = d.link_ptr[I.ado_base + b.ado + 1];
+  for (int l = l0; l < l1; ++l) {
+    const LinkFlat q = d.flat[l];
+    const C x = in[I.elem_base + static_cast<long long>(q.nbr) * N2 + e];
+    const int tb = q.u * N2 + e;
+    C cf;
+    if (I.real_v) {
+      const z128 p = d.cmac_tab[I.tab_off + tb];
+      cf.re = static_cast<R>(p.re) * static_cast<R>(q.iphi.re) +
+              static_cast<R>(p.im) * static_cast<R>(q.psi.re) + static_cast<R>(q.dir.re);
+      cf.im = static_cast<R>(p.re) * static_cast<R>(q.iphi.im) +
+              static_cast<R>(p.im) * static_cast<R>(q.psi.im) + static_cast<R>(q.dir.im);
+    } else {
+      cf = conv<R>(d.cmac_tab[I.tab_off + 2 * tb]) * conv<R>(q.iphi) +
+           conv<R>(d.cmac_tab[I.tab_off + 2 * tb + 1]) * conv<R>(q.psi) + conv<R>(q.dir);
     }
-    err_acc += epilogue<F64, R, KP>(pv, A, cur, gi, D, alpha);
+    cmac(D, cf, x);
   }
-  __syncthreads();
+  err_acc += epilogue<F64, R, KP>(b.pv, A, cur, b.gi, D, alpha);
 }
 
 // KP = 4 for the stages with at most four epilogue operands (every RK4 stage):
 // the smaller operand set keeps 7 waves per SIMD (FP32) instead of 4-5
-template <typename R, bool F64, int KP>
+template <typename R, bool F64, int KP, int IPW>
 __global__ void __launch_bounds__(64) batch_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
-  __shared__ C s_rho[64];
-  __shared__ C s_H[64];
+  __shared__ C s_rho[IPW][64];
+  __shared__ C s_H[IPW][64];
   const Prologue pg = prologue(A);
   if (pg.skip) return;
   const int cur = pg.cur;
   const C* in = static_cast<const C*>(pick(A.in, cur));
-  const auto items = cst(d.items);
-  const auto insts = cst(d.inst);
   double err_acc = 0.0;
-  for (int it = blockIdx.x; it < d.n_items; it += gridDim.x) {
-    const int m = items[it].inst, ado0 = items[it].ado;
-    BatchInst I;
-    I.N = insts[m].N;
-    I.N2 = insts[m].N2;
-    I.n_hrchy = insts[m].n_hrchy;
-    I.apw = insts[m].apw;
-    I.ado_base = insts[m].ado_base;
-    I.h_off = insts[m].h_off;
-    I.tab_off = insts[m].tab_off;
-    I.dterm_off = insts[m].dterm_off;
-    I.real_v = insts[m].real_v;
-    I.elem_base = insts[m].elem_base;
-    batch_item<R, F64, KP>(d, A, in, I, ado0, cur, pg.alpha, s_rho, s_H, err_acc);
+  for (int it = blockIdx.x * IPW; it < d.n_items; it += gridDim.x * IPW) {
+    BatchLane<R, KP> b[IPW];
+#pragma unroll
+    for (int q = 0; q < IPW; ++q)
+      batch_load<R, F64, KP>(d, A, in, it + q, cur, s_rho[q], s_H[q], b[q]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < IPW; ++q)
+      batch_compute<R, F64, KP>(d, A, in, cur, pg.alpha, s_rho[q], s_H[q], b[q], err_acc);
+    __syncthreads();
   }
   if (A.do_err) write_partial(A, err_acc);
 }
@@ -2220,12 +2255,17 @@ int launch_hilb_mfma_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
     throw HeomError{HEOM_ERR_UNSUPPORTED, "matrix-core path: too many noises for LDS"};
   const int grid = cap_grid(d.n_hrchy);
   const bool small = A.npre <= 4;
+  // FP64 and split-bf16 at N = 64: eight waves per ADO, each with half the
+  // output tiles (two workgroups per CU either way, so twice the waves work on
+  // the same ADOs; measured at 969 ADOs: 135 against 158 us per FP64
+  // Dormand-Prince stage, 99 against 138 us split-bf16; complex64 126 us both)
+  constexpr int BS = (N == 64 && (sizeof(R) == 8 || BF16X3)) ? 512 : kMfmaBS;
 #define HAMD_MFMA(REALH, KP, PAD)                                                \
   {                                                                              \
-    auto k = hilb_mfma_kernel<N, R, REALH, F64, KP, BF16X3, PAD>;                \
+    auto k = hilb_mfma_kernel<N, R, REALH, F64, KP, BF16X3, PAD, BS>;            \
     static bool once = (allow_lds(k, kMfmaLdsCap), true);                        \
     (void)once;                                                                  \
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kMfmaBS), lds, st, d, A);             \
+    hipLaunchKernelGGL(k, dim3(grid), dim3(BS), lds, st, d, A);                  \
   }
 #define HAMD_MFMA_P(REALH, KP)                          \
   {                                                     \
@@ -2263,12 +2303,15 @@ int launch_hilb_mfma_dense_n(const FormDev& d, const StageArgs& A, hipStream_t s
     throw HeomError{HEOM_ERR_UNSUPPORTED, "matrix-core path: the blocks exceed LDS"};
   const int grid = cap_grid(d.n_hrchy);
   const bool small = A.npre <= 4;
+  // N = 64: eight waves per ADO (measured at 969 ADOs: 497 against 715 us per
+  // FP64 stage, 309 against 364 us complex64)
+  constexpr int BS = N == 64 ? 512 : kMfmaBS;
 #define HAMD_MD(REALV, REALH, KP, PAD)                                                 \
   {                                                                                    \
-    auto k = hilb_mfma_dense_kernel<N, R, REALV, REALH, F64, KP, PAD>;                 \
+    auto k = hilb_mfma_dense_kernel<N, R, REALV, REALH, F64, KP, PAD, BS>;             \
     static bool once = (allow_lds(k, kMfmaLdsCap), true);                              \
     (void)once;                                                                        \
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kMfmaBS), lds, st, d, A);                   \
+    hipLaunchKernelGGL(k, dim3(grid), dim3(BS), lds, st, d, A);                        \
   }
 #define HAMD_MD_P(REALV, REALH, KP) \
   {                                 \
@@ -2379,10 +2422,20 @@ int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
     return launch_redfield<R, F64>(d, A, st);
   if (d.kind == kBatch) {
     const int grid = d.batch_grid > 0 ? d.batch_grid : 1;
-    if (A.npre <= 4)
-      hipLaunchKernelGGL((batch_kernel<R, F64, 4>), dim3(grid), dim3(64), 0, st, d, A);
-    else
-      hipLaunchKernelGGL((batch_kernel<R, F64, kMaxPre>), dim3(grid), dim3(64), 0, st, d, A);
+    const bool small = A.npre <= 4;
+    if (d.batch_ipw == 2) {
+      if (small)
+        hipLaunchKernelGGL((batch_kernel<R, F64, 4, 2>), dim3(grid), dim3(64), 0, st, d, A);
+      else
+        hipLaunchKernelGGL((batch_kernel<R, F64, kMaxPre, 2>), dim3(grid), dim3(64), 0, st, d,
+                           A);
+    } else {
+      if (small)
+        hipLaunchKernelGGL((batch_kernel<R, F64, 4, 1>), dim3(grid), dim3(64), 0, st, d, A);
+      else
+        hipLaunchKernelGGL((batch_kernel<R, F64, kMaxPre, 1>), dim3(grid), dim3(64), 0, st, d,
+                           A);
+    }
     return grid;
   }
   if (d.kind == 1) {

@@ -128,6 +128,11 @@ class Qme {
   DevBuf d_inst, d_items;
   DevBuf d_Hre, d_Him, d_Hre32, d_Him32;
   DevBuf d_Vre, d_Vim, d_Vre32, d_Vim32;  // dense V planes of the matrix-core path
+  struct TuneEntry {
+    int ipw, grid;
+    float us;  // mean microseconds per derivative pass
+  };
+  std::vector<TuneEntry> tune_log;  // the batch autotuner's candidates, in the order tried
   void autotune_batch();
 
  private:

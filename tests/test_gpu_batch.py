@@ -89,6 +89,37 @@ def test_batch_calc_diff_full_sweep():
     assert worst <= TOL
 
 
+def test_batch_tile_shapes_agree_and_match_the_oracle():
+    """The autotuner times both tile shapes (one or two work items per wavefront
+    step) over several grids and keeps the fastest; every shape and grid must
+    give the same derivative to rounding (the same expression per element; the
+    compiler may contract a product and a sum into one fma differently in the
+    two kernel variants), equal to the oracle's member by member."""
+    ms = diag_models(23, [2, 4, 8, 3, 8, 2, 6, 4, 5])
+    b = H.heom_batch([member(m) for m in ms])
+    chosen, tried = b.tuning()
+    print("autotuner:", chosen, tried, flush=True)
+    assert {t[0] for t in tried} == {1, 2}
+    assert all(t[2] > 0 for t in tried)
+    best = min(t[2] for t in tried)
+    assert (chosen["items_per_step"], chosen["grid"]) in [t[:2] for t in tried if t[2] == best]
+    rng = np.random.default_rng(8)
+    rho = rng.normal(size=b.main_size()) + 1j * rng.normal(size=b.main_size())
+    outs = []
+    for ipw, grid in [(1, 1), (1, 7), (2, 1), (2, 5), (2, 4096), (1, 4096)]:
+        b.set_tuning(ipw, grid)
+        outs.append(b.calc_diff_impl(None, np.zeros_like(rho), rho, 1.0, 0.0))
+    scale = np.abs(outs[0]).max()
+    for o in outs[1:]:
+        assert np.abs(o - outs[0]).max() <= 1e-14 * scale
+    for o in outs:
+        for m, r, g in zip(ms, b.unpack(rho), b.unpack(o)):
+            ref = oracle(m).calc_diff(r, 1.0, 0.0)
+            assert close(g, ref), m["name"]
+    with pytest.raises(H.HeomError):
+        b.set_tuning(3, 16)
+
+
 def test_batch_rk4_sweep_rho_s_every_output():
     """24 sweep instances, RK4 dt=0.005 x 400 steps, rho_S every 100 steps.
 

@@ -235,8 +235,8 @@ def bench_sweep(H, L, steps, warmup, reps):
         L.heom_dev_free(d)
         chosen, tried = b.tuning()
         res[dtype] = {"value": st["ado_stage_updates"] / el, "ms_per_step": 1e3 * el / steps,
-                      "autotune": dict(chosen, tried_us_per_pass={"%dx%d" % (k, g): round(u, 1)
-                                                                  for k, g, u in tried}),
+                      "autotune": dict(chosen, tried_us_per_pass={
+                          "ipw%d_wpb%d_grid%d" % (k, w, g): round(u, 1) for k, w, g, u in tried}),
                       "stage_ms": float(sms[:n].mean()), "stage_bytes": float(sby[:n].mean()),
                       "hbm_frac": gbs / HBM_PEAK_GBS, "rs": b.split_rho_s(rs),
                       "final": b.unpack(final),

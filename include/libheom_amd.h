@@ -163,16 +163,19 @@ int heom_batch_create(const heom_handle* members, int n_members, heom_handle* ou
 int heom_batch_layout(heom_handle batch, long long* elem_base, int* rho_s_offset,
                       int* n_level);
 /* The batch kernel's autotuner (no counterpart in the reference): at
- * heom_batch_create it times one derivative pass of the whole batch for every
- * tile shape (work items per wavefront step, 1 or 2; a work item is 64/n_level^2
- * ADOs of one member) and grid size, and keeps the fastest.  Writes the chosen
- * shape and grid, and up to `cap` timed candidates (items per step, grid, mean
- * microseconds per pass) with their count in *n_tried. */
-int heom_batch_tuning(heom_handle batch, int* items_per_step, int* grid, int* tried_ipw,
-                      int* tried_grid, float* tried_us, int cap, int* n_tried);
-/* Overrides the autotuner's choice (items_per_step 1 or 2, 1 <= grid <= 262144); the
- * results do not depend on the shape, only the speed does. */
-int heom_batch_set_tuning(heom_handle batch, int items_per_step, int grid);
+ * heom_batch_create it times one Runge-Kutta stage pass of the whole batch for
+ * every tile shape -- work items per wavefront step (1 or 2; a work item is
+ * 64/n_level^2 ADOs of one member) and wavefronts per workgroup (1, 2 or 4) --
+ * and grid size, and keeps the fastest.  shape receives {items per step,
+ * wavefronts per workgroup, grid}; tried receives up to `cap` such triples of
+ * the candidates in the order timed, tried_us their microseconds per pass, and
+ * *n_tried their count. */
+int heom_batch_tuning(heom_handle batch, int* shape, int* tried, float* tried_us, int cap,
+                      int* n_tried);
+/* Overrides the autotuner's choice (items_per_step 1 or 2, waves_per_group 1,
+ * 2 or 4, 1 <= grid <= 262144 / waves_per_group); the results do not depend on
+ * the shape beyond rounding, only the speed does. */
+int heom_batch_set_tuning(heom_handle batch, int items_per_step, int waves_per_group, int grid);
 
 /* Learned truncation closure (no counterpart in the reference, which drops
  * every ADO beyond max_depth; heom.h:77-91).  A 3-layer network (n_in ->

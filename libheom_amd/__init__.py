@@ -345,22 +345,26 @@ class heom_batch:
 
     def tuning(self):
         """The autotuner's choice and every candidate it timed:
-        ({"items_per_step": k, "grid": g}, [(items_per_step, grid, us_per_pass), ...])."""
+        ({"items_per_step": k, "waves_per_group": w, "grid": g},
+         [(items_per_step, waves_per_group, grid, us_per_pass), ...])."""
         L = _capi.load()
         cap = 64
-        ipw, grid, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        ti = np.zeros(cap, np.int32)
-        tg = np.zeros(cap, np.int32)
-        tu = np.zeros(cap, np.float32)
-        _capi.check(L.heom_batch_tuning(self._h, ctypes.byref(ipw), ctypes.byref(grid), _ptr(ti),
-                                        _ptr(tg), _ptr(tu), cap, ctypes.byref(n)))
+        shape = np.zeros(3, np.int32)
+        tried = np.zeros(3 * cap, np.int32)
+        us = np.zeros(cap, np.float32)
+        n = ctypes.c_int()
+        _capi.check(L.heom_batch_tuning(self._h, _ptr(shape), _ptr(tried), _ptr(us), cap,
+                                        ctypes.byref(n)))
         k = min(n.value, cap)
-        return ({"items_per_step": ipw.value, "grid": grid.value},
-                [(int(ti[i]), int(tg[i]), float(tu[i])) for i in range(k)])
+        return ({"items_per_step": int(shape[0]), "waves_per_group": int(shape[1]),
+                 "grid": int(shape[2])},
+                [(int(tried[3 * i]), int(tried[3 * i + 1]), int(tried[3 * i + 2]), float(us[i]))
+                 for i in range(k)])
 
-    def set_tuning(self, items_per_step, grid):
-        """Override the autotuner's tile shape and grid (results do not change)."""
-        _capi.check(_capi.load().heom_batch_set_tuning(self._h, int(items_per_step), int(grid)))
+    def set_tuning(self, items_per_step, waves_per_group, grid):
+        """Override the autotuner's tile shape and grid (results change only by rounding)."""
+        _capi.check(_capi.load().heom_batch_set_tuning(self._h, int(items_per_step),
+                                                       int(waves_per_group), int(grid)))
 
     def rho_s_size(self):
         return int(np.sum(self.n_levels.astype(np.int64) ** 2))

@@ -102,8 +102,8 @@ def load():
         "heom_calc_diff_host": (ip, [vp, vp, vp, dp, dp, dp, dp]),
         "heom_batch_create": (ip, [vp, ip, P(vp)]),
         "heom_batch_layout": (ip, [vp, vp, vp, vp]),
-        "heom_batch_tuning": (ip, [vp, vp, vp, vp, vp, vp, ip, vp]),
-        "heom_batch_set_tuning": (ip, [vp, ip, ip]),
+        "heom_batch_tuning": (ip, [vp, vp, vp, vp, ip, vp]),
+        "heom_batch_set_tuning": (ip, [vp, ip, ip, ip]),
         "heom_solver_create": (ip, [vp, ip, P(vp)]),
         "heom_solver_destroy": (ip, [vp]),
         "heom_solve": (ip, [vp, vp, vp, ip, CALLBACK, vp, dp, dp, dp, ip]),

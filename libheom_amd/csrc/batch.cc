@@ -102,62 +102,85 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
   dev.items = d_items.as<BatchItem>();
   dev.batch_grid = std::min(dev.n_items, kMaxGrid);
   dev.batch_ipw = 1;
+  dev.batch_wpb = 1;
   ready = true;
   autotune_batch();
 }
 
-// Autotuner of the persistent batch kernel: time one derivative evaluation of
-// the whole batch for every tile shape (work items per wavefront step, 1 or 2)
-// and grid size (512 .. 16384 workgroups, or one per item group), keep the
-// fastest.  The results are the same for every choice (each element's
-// arithmetic does not depend on it).
+// Autotuner of the persistent batch kernel: time one pass of a Runge-Kutta
+// stage of the whole batch (operator, two FP64 operands, two outputs: the
+// shape of an RK4 stage) for every tile shape -- work items per wavefront step
+// (1 or 2) and wavefronts per workgroup (1, 2 or 4) -- and grid size (4096 or
+// 16384 persistent workgroups, or one per item group), and keep the fastest.  The results do not depend on the choice beyond rounding
+// (the same expression per element).
 void Qme::autotune_batch() {
   const long long ms = main_size();
   const size_t eb = lo() ? sizeof(z64) : sizeof(z128);
-  DevBuf a, b;
+  DevBuf a, b, y, acc, acc2;
   a.alloc(eb * ms);
   b.alloc(eb * ms);
+  y.alloc(sizeof(z128) * ms);
+  acc.alloc(sizeof(z128) * ms);
+  acc2.alloc(sizeof(z128) * ms);
   HAMD_HIP(hipMemsetAsync(a.p, 0, eb * ms, stream));
+  HAMD_HIP(hipMemsetAsync(y.p, 0, sizeof(z128) * ms, stream));
+  HAMD_HIP(hipMemsetAsync(acc.p, 0, sizeof(z128) * ms, stream));
   StageArgs A;
   std::memset(&A, 0, sizeof(A));
-  A.mode = MODE_DIFF;
+  A.mode = MODE_STAGE;
   A.in[0] = A.in[1] = a.p;
   A.in_lo = lo();
-  A.dst[0][0] = A.dst[0][1] = b.p;
-  A.dst_lo[0] = A.in_lo;
-  A.a_re = 1.0;
+  A.a_re = 1e-3;
+  A.npre = 2;
+  for (int c = 0; c < 2; ++c) {
+    A.pre[c][0] = y.p;
+    A.pre[c][1] = acc.p;
+    A.dst[0][c] = b.p;
+    A.dst[1][c] = acc2.p;
+  }
+  A.ncombo = 2;
+  A.cc[0][0] = 1.0;
+  A.ck[0] = 0.5;
+  A.cc[1][1] = 1.0;
+  A.ck[1] = 1.0 / 6.0;
+  A.dst_lo[0] = lo();
   const int rb = lo() ? 4 : 8;
   hipEvent_t e0, e1;
   HAMD_HIP(hipEventCreate(&e0));
   HAMD_HIP(hipEventCreate(&e1));
   float best = std::numeric_limits<float>::max();
-  int best_g = std::min(dev.n_items, kMaxGrid), best_ipw = 1;
+  int best_g = std::min(dev.n_items, kMaxGrid), best_ipw = 1, best_wpb = 1;
   tune_log.clear();
   for (int ipw : {1, 2}) {
-    const int groups = (dev.n_items + ipw - 1) / ipw;
-    std::vector<int> cands;
-    for (int g : {512, 1024, 2048, 4096, 8192, 16384})
-      if (g < groups) cands.push_back(g);
-    cands.push_back(std::min(groups, kMaxGrid));
-    for (int g : cands) {
-      dev.batch_grid = g;
-      dev.batch_ipw = ipw;
-      launch_operator(dev, rb, A, stream);  // warm-up
-      float shape_best = std::numeric_limits<float>::max();
-      for (int round = 0; round < 3; ++round) {  // best of three rounds of five passes
-        HAMD_HIP(hipEventRecord(e0, stream));
-        for (int r = 0; r < 5; ++r) launch_operator(dev, rb, A, stream);
-        HAMD_HIP(hipEventRecord(e1, stream));
-        HAMD_HIP(hipEventSynchronize(e1));
-        float ms_ = 0.f;
-        HAMD_HIP(hipEventElapsedTime(&ms_, e0, e1));
-        shape_best = std::min(shape_best, ms_);
-      }
-      tune_log.push_back(TuneEntry{ipw, g, 1e3f * shape_best / 5.0f});
-      if (shape_best < best) {
-        best = shape_best;
-        best_g = g;
-        best_ipw = ipw;
+    for (int wpb : {1, 2, 4}) {
+      // one workgroup per group of ipw * wpb items, or fewer persistent ones
+      const int groups = std::min((dev.n_items + ipw * wpb - 1) / (ipw * wpb), kMaxGrid / wpb);
+      std::vector<int> cands;
+      for (int g : {4096, 16384})
+        if (g < groups) cands.push_back(g);
+      cands.push_back(groups);
+      for (int g : cands) {
+        dev.batch_grid = g;
+        dev.batch_ipw = ipw;
+        dev.batch_wpb = wpb;
+        launch_operator(dev, rb, A, stream);  // warm-up
+        float shape_best = std::numeric_limits<float>::max();
+        for (int round = 0; round < 3; ++round) {  // best of three rounds of five passes
+          HAMD_HIP(hipEventRecord(e0, stream));
+          for (int r = 0; r < 5; ++r) launch_operator(dev, rb, A, stream);
+          HAMD_HIP(hipEventRecord(e1, stream));
+          HAMD_HIP(hipEventSynchronize(e1));
+          float ms_ = 0.f;
+          HAMD_HIP(hipEventElapsedTime(&ms_, e0, e1));
+          shape_best = std::min(shape_best, ms_);
+        }
+        tune_log.push_back(TuneEntry{ipw, wpb, g, 1e3f * shape_best / 5.0f});
+        if (shape_best < best) {
+          best = shape_best;
+          best_g = g;
+          best_ipw = ipw;
+          best_wpb = wpb;
+        }
       }
     }
   }
@@ -165,6 +188,7 @@ void Qme::autotune_batch() {
   (void)hipEventDestroy(e1);
   dev.batch_grid = best_g;
   dev.batch_ipw = best_ipw;
+  dev.batch_wpb = best_wpb;
 }
 
 }  // namespace hamd

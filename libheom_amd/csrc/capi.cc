@@ -331,17 +331,23 @@ int heom_batch_layout(heom_handle batch, long long* elem_base, int* rho_s_offset
   });
 }
 
-int heom_batch_tuning(heom_handle batch, int* items_per_step, int* grid, int* tried_ipw,
-                      int* tried_grid, float* tried_us, int cap, int* n_tried) {
+int heom_batch_tuning(heom_handle batch, int* shape, int* tried, float* tried_us, int cap,
+                      int* n_tried) {
   return guard([&] {
     hamd::Qme& q = Qready(batch);
     if (q.kind != hamd::kBatch) throw hamd::HeomError{HEOM_ERR_ARG, "not a batch handle"};
-    if (items_per_step) *items_per_step = q.dev.batch_ipw;
-    if (grid) *grid = q.dev.batch_grid;
+    if (shape) {
+      shape[0] = q.dev.batch_ipw;
+      shape[1] = q.dev.batch_wpb;
+      shape[2] = q.dev.batch_grid;
+    }
     const int n = static_cast<int>(q.tune_log.size());
     for (int k = 0; k < n && k < cap; ++k) {
-      if (tried_ipw) tried_ipw[k] = q.tune_log[k].ipw;
-      if (tried_grid) tried_grid[k] = q.tune_log[k].grid;
+      if (tried) {
+        tried[3 * k] = q.tune_log[k].ipw;
+        tried[3 * k + 1] = q.tune_log[k].wpb;
+        tried[3 * k + 2] = q.tune_log[k].grid;
+      }
       if (tried_us) tried_us[k] = q.tune_log[k].us;
     }
     if (n_tried) *n_tried = n;
@@ -349,14 +355,18 @@ int heom_batch_tuning(heom_handle batch, int* items_per_step, int* grid, int* tr
   });
 }
 
-int heom_batch_set_tuning(heom_handle batch, int items_per_step, int grid) {
+int heom_batch_set_tuning(heom_handle batch, int items_per_step, int waves_per_group, int grid) {
   return guard([&] {
     hamd::Qme& q = Qready(batch);
     if (q.kind != hamd::kBatch) throw hamd::HeomError{HEOM_ERR_ARG, "not a batch handle"};
-    if ((items_per_step != 1 && items_per_step != 2) || grid < 1 || grid > hamd::kMaxGrid)
-      throw hamd::HeomError{HEOM_ERR_ARG, "heom_batch_set_tuning: items_per_step must be 1 or 2 "
-                                          "and grid in 1 .. 262144"};
+    if ((items_per_step != 1 && items_per_step != 2) ||
+        (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4) || grid < 1 ||
+        grid > hamd::kMaxGrid / waves_per_group)
+      throw hamd::HeomError{HEOM_ERR_ARG,
+                            "heom_batch_set_tuning: items_per_step 1 or 2, waves_per_group 1, 2 "
+                            "or 4, 1 <= grid <= 262144 / waves_per_group"};
     q.dev.batch_ipw = items_per_step;
+    q.dev.batch_wpb = waves_per_group;
     q.dev.batch_grid = grid;
     return HEOM_OK;
   });

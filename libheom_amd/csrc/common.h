@@ -237,6 +237,7 @@ struct FormDev {
   // batch
   int n_inst, n_items, batch_grid;
   int batch_ipw;  // work items per wavefront step (1 or 2), picked by the autotuner
+  int batch_wpb;  // wavefronts per workgroup (1, 2 or 4), picked by the autotuner
   const BatchInst* inst;
   const BatchItem* items;
 };

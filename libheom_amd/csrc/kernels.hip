@@ -298,6 +298,26 @@ inline size_t fast_vd_lds(const FormDev& d, bool real, int real_bytes) {
   return rows * (real ? 1 : 2) * static_cast<size_t>(real_bytes);
 }
 
+// Partial sum of the error norm of one wavefront (no workgroup barrier),
+// written to slot `slot`.
+__device__ void write_partial_wave(const StageArgs& A, double v, int slot) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) A.partials[slot] = v;
+}
+
+// LDS written by a wavefront, then read by other lanes of the same wavefront:
+// the LDS serves one wavefront's operations in order, so only the compiler
+// must keep them in order.
+// The scheduling barrier keeps the code on either side apart, as a workgroup
+// barrier would (without it the compiler hoists later loads above the LDS
+// phase and holds their registers across it).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 template <int N, typename R, bool REAL, bool F64, int KP>
 __global__ void __launch_bounds__(WaveCfg<N>::BS, (KP > 4 && WaveCfg<N>::BS == 64) ? 5 : 1)
     hilb_fast_kernel(const FormDev d, const StageArgs A) {
@@ -1340,10 +1360,9 @@ struct BatchLane {
 // and the item's H and ADO blocks in LDS.
 template <typename R, bool F64, int KP>
 __device__ __forceinline__ void batch_load(const FormDev& d, const StageArgs& A, const cx<R>* in,
-                                           int it, int cur, cx<R>* s_rho, cx<R>* s_H,
+                                           int it, int cur, int lane, cx<R>* s_rho, cx<R>* s_H,
                                            BatchLane<R, KP>& b) {
   using C = cx<R>;
-  const int lane = threadIdx.x;
   const auto items = cst(d.items);
   const auto insts = cst(d.inst);
   const bool live = it < d.n_items;
@@ -1376,7 +1395,7 @@ __device__ __forceinline__ void batch_load(const FormDev& d, const StageArgs& A,
 // Compute phase: D of the lane's element and the Runge-Kutta epilogue.
 template <typename R, bool F64, int KP>
 __device__ __forceinline__ void batch_compute(const FormDev& d, const StageArgs& A,
-                                              const cx<R>* in, int cur, z128 alpha,
+                                              const cx<R>* in, int cur, int lane, z128 alpha,
                                               const cx<R>* s_rho, const cx<R>* s_H,
                                               const BatchLane<R, KP>& b, double& err_acc) {
   using C = cx<R>;
@@ -1384,7 +1403,7 @@ __device__ __forceinline__ void batch_compute(const FormDev& d, const StageArgs&
   const BatchInst& I = b.I;
   const int N = I.N, N2 = I.N2, i = b.i, j = b.j, e = b.e;
   C acc{0, 0};
-  const C* bl = s_rho + (threadIdx.x / N2) * N2;
+  const C* bl = s_rho + (lane / N2) * N2;
   // members have N <= 8: unrolled to 8 with an exit (measured faster than the
   // rolled loop in FP64)
 #pragma unroll
@@ -1418,29 +1437,47 @@ __device__ __forceinline__ void batch_compute(const FormDev& d, const StageArgs&
 }
 
 // KP = 4 for the stages with at most four epilogue operands (every RK4 stage):
-// the smaller operand set keeps 7 waves per SIMD (FP32) instead of 4-5
-template <typename R, bool F64, int KP, int IPW>
-__global__ void __launch_bounds__(64) batch_kernel(const FormDev d, const StageArgs A) {
+// the smaller operand set keeps 7 waves per SIMD (FP32) instead of 4-5.
+// WPB wavefronts per workgroup, each with its own items and LDS region and
+// none waiting for another (whether that pays is the autotuner's call); the
+// error-norm partial sums are one per wavefront.
+template <typename R, bool F64, int KP, int IPW, int WPB>
+__global__ void __launch_bounds__(64 * WPB) batch_kernel(const FormDev d, const StageArgs A) {
   using C = cx<R>;
-  __shared__ C s_rho[IPW][64];
-  __shared__ C s_H[IPW][64];
+  __shared__ C s_rho[WPB][IPW][64];
+  __shared__ C s_H[WPB][IPW][64];
   const Prologue pg = prologue(A);
   if (pg.skip) return;
   const int cur = pg.cur;
   const C* in = static_cast<const C*>(pick(A.in, cur));
+  const int w = WPB > 1 ? __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / 64) : 0;
+  const int lane = static_cast<int>(threadIdx.x) - 64 * w;
   double err_acc = 0.0;
-  for (int it = blockIdx.x * IPW; it < d.n_items; it += gridDim.x * IPW) {
+  for (int it = (static_cast<int>(blockIdx.x) * WPB + w) * IPW; it < d.n_items;
+       it += gridDim.x * WPB * IPW) {
     BatchLane<R, KP> b[IPW];
 #pragma unroll
     for (int q = 0; q < IPW; ++q)
-      batch_load<R, F64, KP>(d, A, in, it + q, cur, s_rho[q], s_H[q], b[q]);
-    __syncthreads();
+      batch_load<R, F64, KP>(d, A, in, it + q, cur, lane, s_rho[w][q], s_H[w][q], b[q]);
+    if constexpr (WPB > 1)
+      wave_lds_sync();
+    else
+      __syncthreads();
 #pragma unroll
     for (int q = 0; q < IPW; ++q)
-      batch_compute<R, F64, KP>(d, A, in, cur, pg.alpha, s_rho[q], s_H[q], b[q], err_acc);
-    __syncthreads();
+      batch_compute<R, F64, KP>(d, A, in, cur, lane, pg.alpha, s_rho[w][q], s_H[w][q], b[q],
+                                err_acc);
+    if constexpr (WPB > 1)
+      wave_lds_sync();
+    else
+      __syncthreads();
   }
-  if (A.do_err) write_partial(A, err_acc);
+  if (A.do_err) {
+    if constexpr (WPB > 1)
+      write_partial_wave(A, err_acc, static_cast<int>(blockIdx.x) * WPB + w);
+    else
+      write_partial(A, err_acc);
+  }
 }
 
 // closure network input (Re, Im of the last-tier ADOs of the stage input)
@@ -2421,22 +2458,29 @@ int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
   if (d.kind == HEOM_REDFIELD_HILB || d.kind == HEOM_REDFIELD_LIOU)
     return launch_redfield<R, F64>(d, A, st);
   if (d.kind == kBatch) {
-    const int grid = d.batch_grid > 0 ? d.batch_grid : 1;
+    const int wpb = d.batch_wpb > 0 ? d.batch_wpb : 1;
+    // the persistent loop covers every item at any grid; keep one partial slot per wavefront
+    const int grid = std::max(1, std::min(d.batch_grid, kMaxGrid / wpb));
     const bool small = A.npre <= 4;
-    if (d.batch_ipw == 2) {
-      if (small)
-        hipLaunchKernelGGL((batch_kernel<R, F64, 4, 2>), dim3(grid), dim3(64), 0, st, d, A);
-      else
-        hipLaunchKernelGGL((batch_kernel<R, F64, kMaxPre, 2>), dim3(grid), dim3(64), 0, st, d,
-                           A);
-    } else {
-      if (small)
-        hipLaunchKernelGGL((batch_kernel<R, F64, 4, 1>), dim3(grid), dim3(64), 0, st, d, A);
-      else
-        hipLaunchKernelGGL((batch_kernel<R, F64, kMaxPre, 1>), dim3(grid), dim3(64), 0, st, d,
-                           A);
-    }
-    return grid;
+#define HAMD_BATCH(IPW, WPB)                                                               \
+  {                                                                                        \
+    if (small)                                                                             \
+      hipLaunchKernelGGL((batch_kernel<R, F64, 4, IPW, WPB>), dim3(grid), dim3(64 * WPB), 0, \
+                         st, d, A);                                                        \
+    else                                                                                   \
+      hipLaunchKernelGGL((batch_kernel<R, F64, kMaxPre, IPW, WPB>), dim3(grid),            \
+                         dim3(64 * WPB), 0, st, d, A);                                     \
+  }
+#define HAMD_BATCH_W(IPW)                   \
+  {                                         \
+    if (wpb == 4) HAMD_BATCH(IPW, 4)        \
+    else if (wpb == 2) HAMD_BATCH(IPW, 2)   \
+    else HAMD_BATCH(IPW, 1)                 \
+  }
+    if (d.batch_ipw == 2) HAMD_BATCH_W(2) else HAMD_BATCH_W(1)
+#undef HAMD_BATCH_W
+#undef HAMD_BATCH
+    return grid * wpb;  // error-norm partials: one per wavefront
   }
   if (d.kind == 1) {
     const int APB = 256 / d.N2;

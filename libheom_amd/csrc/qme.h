@@ -129,8 +129,8 @@ class Qme {
   DevBuf d_Hre, d_Him, d_Hre32, d_Him32;
   DevBuf d_Vre, d_Vim, d_Vre32, d_Vim32;  // dense V planes of the matrix-core path
   struct TuneEntry {
-    int ipw, grid;
-    float us;  // mean microseconds per derivative pass
+    int ipw, wpb, grid;
+    float us;  // mean microseconds per stage pass
   };
   std::vector<TuneEntry> tune_log;  // the batch autotuner's candidates, in the order tried
   void autotune_batch();

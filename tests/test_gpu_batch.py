@@ -90,24 +90,27 @@ def test_batch_calc_diff_full_sweep():
 
 
 def test_batch_tile_shapes_agree_and_match_the_oracle():
-    """The autotuner times both tile shapes (one or two work items per wavefront
-    step) over several grids and keeps the fastest; every shape and grid must
-    give the same derivative to rounding (the same expression per element; the
-    compiler may contract a product and a sum into one fma differently in the
-    two kernel variants), equal to the oracle's member by member."""
+    """The autotuner times every tile shape (one or two work items per wavefront
+    step, one, two or four wavefronts per workgroup) over several grids and
+    keeps the fastest; every shape and grid must give the same derivative to
+    rounding (the same expression per element; the compiler may contract a
+    product and a sum into one fma differently in the kernel variants), equal
+    to the oracle's member by member."""
     ms = diag_models(23, [2, 4, 8, 3, 8, 2, 6, 4, 5])
     b = H.heom_batch([member(m) for m in ms])
     chosen, tried = b.tuning()
     print("autotuner:", chosen, tried, flush=True)
-    assert {t[0] for t in tried} == {1, 2}
-    assert all(t[2] > 0 for t in tried)
-    best = min(t[2] for t in tried)
-    assert (chosen["items_per_step"], chosen["grid"]) in [t[:2] for t in tried if t[2] == best]
+    assert {t[0] for t in tried} == {1, 2} and {t[1] for t in tried} == {1, 2, 4}
+    assert all(t[3] > 0 for t in tried)
+    best = min(t[3] for t in tried)
+    assert (chosen["items_per_step"], chosen["waves_per_group"], chosen["grid"]) in \
+        [t[:3] for t in tried if t[3] == best]
     rng = np.random.default_rng(8)
     rho = rng.normal(size=b.main_size()) + 1j * rng.normal(size=b.main_size())
     outs = []
-    for ipw, grid in [(1, 1), (1, 7), (2, 1), (2, 5), (2, 4096), (1, 4096)]:
-        b.set_tuning(ipw, grid)
+    for shape in [(1, 1, 1), (1, 1, 7), (2, 1, 1), (2, 1, 5), (1, 2, 3), (2, 4, 1), (1, 4, 2),
+                  (2, 2, 4096), (1, 1, 4096)]:
+        b.set_tuning(*shape)
         outs.append(b.calc_diff_impl(None, np.zeros_like(rho), rho, 1.0, 0.0))
     scale = np.abs(outs[0]).max()
     for o in outs[1:]:
@@ -117,7 +120,31 @@ def test_batch_tile_shapes_agree_and_match_the_oracle():
             ref = oracle(m).calc_diff(r, 1.0, 0.0)
             assert close(g, ref), m["name"]
     with pytest.raises(H.HeomError):
-        b.set_tuning(3, 16)
+        b.set_tuning(3, 1, 16)
+    with pytest.raises(H.HeomError):
+        b.set_tuning(1, 3, 16)
+
+
+def test_batch_rkdp_with_several_wavefronts_per_workgroup():
+    """Dormand-Prince on 7 copies of the well-conditioned spin-boson of
+    test_batch_rkdp_identical_copies_match_single_solve, in every tile shape:
+    the error norm's partial sums are one per wavefront; the accepted and
+    rejected attempts must be the oracle's and rho_S within the FP64 bar."""
+    m = configs.spin_boson(max_depth=6, T=0.5)
+    m["t_list"] = np.linspace(0.0, 5.0, 11)
+    kw = {"dt": 0.8, "atol": 1e-10, "rtol": 1e-8}
+    _, rs_o, st_o, _, _ = oracle(m).solve(m["rho0"], m["t_list"], O.RKDP, kw["dt"], kw["atol"],
+                                          kw["rtol"])
+    g = member(m)
+    for shape in [(1, 1, 3), (2, 2, 1), (1, 4, 1), (2, 4, 64)]:
+        b = H.heom_batch([g] * 7)
+        b.set_tuning(*shape)
+        s = H.qme_solver(eng(), b, H.rkdp())
+        _, rs = s.solve_rho_s(b.pack([m["rho0"]] * 7), m["t_list"], kw)
+        st = s.stats()
+        assert (st["n_accepted"], st["n_rejected"]) == (st_o.n_accepted, st_o.n_rejected), shape
+        for got_rs in b.split_rho_s(rs):
+            assert close(got_rs, rs_o), shape
 
 
 def test_batch_rk4_sweep_rho_s_every_output():

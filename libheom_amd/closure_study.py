@@ -12,10 +12,13 @@ models seeds 10000 .. 10000+held_out-1.
    last tier): input (Re, Im of rho_n, n, model features), target the
    tier-4 term that depth-3 truncation drops.
 2. A 3-layer width-256 tanh network is trained on them (Adam, batch 4096).
-3. Held-out models: the fraction of the dropped term the network recovers on
+3. Gain: the network's output scaled by g in {0, 0.25, 0.5, 0.75, 1} on 20
+   validation models (seeds 20000 ..), keeping the g with the lowest median
+   rho_S error against their depth-8 runs (g = 0 is plain truncation).
+4. Held-out models: the fraction of the dropped term the network recovers on
    their deep states, and the rho_S(t) error against the depth-8 run of plain
-   depth 3, depth 3 with the closure, and plain depths 4 and 5, with the time
-   per run.
+   depth 3, depth 3 with the closure (gain 1 and the validated gain), and
+   plain depths 4 and 5, with the time per run.
 
 Prints one JSON object; everything runs on the GPU in the library's kernels.
 """
@@ -73,30 +76,48 @@ def main(argv=None):
     P = net.forward(X).astype(np.float64)
     rec = float(1.0 - np.sum((P - Y) ** 2) / np.sum(Y ** 2))
 
-    def solve(seed, depth, closure=None):
+    xm, xs, ym, ys = net.get_scaling()
+
+    def solve(seed, depth, closure=None, gain=1.0):
         m = configs.chain4(seed, max_depth=depth)
         q = H.heom_hilb(depth).load(m, eng)
-        if closure is not None:
+        if closure is not None and gain > 0.0:  # gain 0: plain truncation
+            # the bound network is a snapshot: its output scaled by the gain
+            net.set_scaling(xm, xs, gain * ym, gain * ys)
             q.set_closure(net, closure)
+            net.set_scaling(xm, xs, ym, ys)
         s = H.qme_solver(eng, q, H.rk4())
         t = time.perf_counter()
         _, rs = s.solve_rho_s(m["rho0"], t_list, {"dt": dt})
         return rs, time.perf_counter() - t
 
-    runs = {"plain depth 3": (L, False), "depth 3 + closure": (L, True),
-            "plain depth 4": (4, False), "plain depth 5": (5, False),
-            "plain depth %d" % a.deep: (a.deep, False)}
+    va_seeds = [20000 + s for s in range(20)]
+    d_va, f_va = data(va_seeds, rows=False)
+    gains = [0.0, 0.25, 0.5, 0.75, 1.0]
+    val = {}
+    for g in gains:
+        e = [float(np.abs(solve(s, L, f_va[i], g)[0] - d_va.rho_s[i]).max())
+             for i, s in enumerate(va_seeds)]
+        val[g] = float(np.median(e))
+        log("validation gain %.2f: median rho_S error %.3e" % (g, val[g]))
+    g_best = min(gains, key=lambda g: val[g])
+
+    runs = {"plain depth 3": (L, None), "depth 3 + closure": (L, 1.0),
+            "depth 3 + closure, validated gain": (L, g_best),
+            "plain depth 4": (4, None), "plain depth 5": (5, None),
+            "plain depth %d" % a.deep: (a.deep, None)}
     err = {k: [] for k in runs}
     tms = {k: [] for k in runs}
     for i, s in enumerate(te_seeds):
-        for k, (depth, clo) in runs.items():
-            rs, el = solve(s, depth, f_te[i] if clo else None)
+        for k, (depth, g) in runs.items():
+            rs, el = solve(s, depth, f_te[i] if g is not None else None, 1.0 if g is None else g)
             err[k].append(float(np.abs(rs - d_te.rho_s[i]).max()))
             tms[k].append(el)
         if i % 10 == 9:
             log("held-out %d / %d" % (i + 1, len(te_seeds)))
     plain = np.array(err["plain depth 3"])
     clo = np.array(err["depth 3 + closure"])
+    clo_g = np.array(err["depth 3 + closure, validated gain"])
     out = {
         "workload": "4-site chain, 8 modes; closure at depth 3 trained on %d plain depth-%d "
                     "runs (RK4 dt=0.01, t=0..10, 20 states each)" % (a.train, a.deep),
@@ -108,6 +129,10 @@ def main(argv=None):
                                     "max": float(np.max(v))} for k, v in err.items()},
         "ms_per_run": {k: 1e3 * float(np.mean(v)) for k, v in tms.items()},
         "closure_better_than_plain_depth3_fraction": float(np.mean(clo < plain)),
+        "validation_median_rho_s_error_by_gain": {"%.2f" % g: v for g, v in val.items()},
+        "validated_gain": g_best,
+        "validated_gain_closure_better_than_plain_depth3_fraction":
+            float(np.mean(clo_g < plain)),
     }
     print(json.dumps(out, indent=1))
     return out

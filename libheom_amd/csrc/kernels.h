@@ -38,6 +38,14 @@ void launch_ctrl_init(Ctrl* c, double t0, double t_bound, double dt, hipStream_t
 void launch_closure_gather(const FormDev& d, const StageArgs& A, float* x, int ado_stride,
                            hipStream_t stream);
 void launch_gather_rho_s(const FormDev& d, void* dst, const void* src, hipStream_t stream);
+// the whole per-ADO closure network in one launch (gather + three layers on
+// the matrix cores): widths n_in <= 64, n_hid a multiple of 16 up to 256,
+// n_out a multiple of 16 up to 64 (closure_mlp_fits)
+bool closure_mlp_fits(int n_in, int n_hid, int n_out);
+void launch_closure_mlp(const FormDev& d, const StageArgs& A, const float* xc, int n_in,
+                        int n_hid, int n_out, const float* W1, const float* b1, const float* W2,
+                        const float* b2, const float* W3, const float* b3, float* y,
+                        hipStream_t stream);
 
 // Resident fixed-step integrator (heom_hilb with diagonal V and at most 8
 // links per ADO, or redfield_hilb with diagonal V; FP64, at most kResidentMax

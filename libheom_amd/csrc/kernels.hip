@@ -1002,16 +1002,17 @@ __global__ void __launch_bounds__(BS, BS / 128)
     // Neighbour couplings and the Runge-Kutta epilogue, element-parallel and
     // coalesced.  Two schedules of the same per-element arithmetic (D, then
     // the links in order, then the epilogue):
-    //  - links outer (FP64 and BF16X3 forms): for each link the loads of LCH of
-    //    the thread's elements of the neighbour block are in flight together;
-    //    each thread updates only its own elements (x = tid + c*BS) in LDS and
-    //    reads them back in the epilogue, so the phases need no barrier;
-    //  - links inside the epilogue passes (complex64 form, whose register
-    //    budget keeps three waves per SIMD that way).
+    //  - links outer (FP64 and BF16X3 forms, and every form with eight waves
+    //    per ADO): for each link the loads of LCH of the thread's elements of
+    //    the neighbour block are in flight together; each thread updates only
+    //    its own elements (x = tid + c*BS) in LDS and reads them back in the
+    //    epilogue, so the phases need no barrier;
+    //  - links inside the epilogue passes (complex64 form with four waves per
+    //    ADO, whose register budget keeps three waves per SIMD that way).
     // Each epilogue pass takes CH elements per thread and issues every load
     // before its stores (the stores may alias later loads); elements past the
     // block read element 0 and store nothing.
-    constexpr bool kLinksOuter = sizeof(R) == 8 || BF16X3;
+    constexpr bool kLinksOuter = sizeof(R) == 8 || BF16X3 || BS > 256;
     constexpr int EPT = N2 / BS;              // elements per thread (tile)
     constexpr int LMAX = BS > 256 ? 4 : 8;    // register budget: 256 / 128 per lane
     constexpr int LCH = EPT < LMAX ? EPT : LMAX;  // elements per load group
@@ -1495,6 +1496,128 @@ __global__ void closure_gather_kernel(const FormDev d, const StageArgs A, float*
     const int e = static_cast<int>(i - r * d.N2);
     x[r * ado_stride + e] = static_cast<float>(v.re);
     x[r * ado_stride + d.N2 + e] = static_cast<float>(v.im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Learned closure, one launch per operator pass (per-ADO layout): each
+// workgroup takes 16 last-tier ADOs of the stage input, gathers their network
+// rows (Re, Im of rho_n, then the constant columns written at binding), and
+// runs the three layers on the matrix cores (f32 16x16x4, FP32 accumulation):
+// h1 = tanh(x W1^T + b1), h2 = tanh(h1 W2^T + b2), y = h2 W3^T + b3 (the
+// normalisation folded into W1, b1, W3, b3).  Rows stay in LDS between layers;
+// the weights (row-major, one row per output unit) are read from L2.
+// ---------------------------------------------------------------------------
+constexpr int kClosRows = 16;   // network rows per workgroup
+constexpr int kClosInMax = 64;  // widest input the fused kernel takes
+constexpr int kClosHidMax = 256;
+
+// acc[t] += (rows of `a` in LDS, leading dimension lda) x W^T for output
+// columns [c0 + 16 t, c0 + 16 t + 16), k over [0, K) (K a multiple of 4;
+// columns of W at or past kw read as zero).  Eight k steps at a time: their
+// operands are loaded before their products, so the weight loads of a group
+// are in flight together.
+template <int T>
+__device__ __forceinline__ void clos_layer(const float* a, int lda, const float* W, int kw, int K,
+                                           int c0, int lr, int lk, mf_f4 (&acc)[T]) {
+  constexpr int G = 8;
+  for (int k0 = 0; k0 < K; k0 += 4 * G) {
+    float av[G], bv[G][T];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + 4 * g + lk;
+      const bool in = k0 + 4 * g < K;
+      av[g] = in ? a[lr * lda + k] : 0.f;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int n = c0 + 16 * t + lr;
+        bv[g][t] = (in && k < kw) ? W[static_cast<long long>(n) * kw + k] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[g], bv[g][t], acc[t], 0, 0, 0);
+  }
+}
+
+__global__ void __launch_bounds__(256) closure_mlp_kernel(const FormDev d, const StageArgs A,
+                                                          const float* xc, int n_in, int n_hid,
+                                                          int n_out, const float* W1,
+                                                          const float* b1, const float* W2,
+                                                          const float* b2, const float* W3,
+                                                          const float* b3, float* y) {
+  constexpr int LDX = kClosInMax + 4, LDH = kClosHidMax + 4;
+  __shared__ float s_x[kClosRows * LDX];
+  __shared__ float s_h1[kClosRows * LDH];
+  __shared__ float s_h2[kClosRows * LDH];
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const void* in = pick(A.in, pg.cur);
+  const int r0 = blockIdx.x * kClosRows;
+  const int rows = d.clos_n, N2 = d.N2;
+  const int kin = (n_in + 3) & ~3;
+  // gather: rho of ADO lo + r, then the constant columns (zero past n_in and rows)
+  for (int x = threadIdx.x; x < kClosRows * kin; x += blockDim.x) {
+    const int r = x / kin, c = x - (x / kin) * kin;
+    float v = 0.f;
+    if (r0 + r < rows && c < n_in) {
+      if (c < 2 * N2) {
+        const long long gi = static_cast<long long>(d.clos_lo + r0 + r) * N2 + (c % N2);
+        const z128 z = ldz(in, A.in_lo != 0, gi);
+        v = static_cast<float>(c < N2 ? z.re : z.im);
+      } else {
+        v = xc[static_cast<long long>(r0 + r) * n_in + c];
+      }
+    }
+    s_x[r * LDX + c] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  // hidden layers: wave w takes columns [64 w, 64 w + 64) (n_hid <= 256)
+  const float* hin = s_x;
+  int lda = LDX, kw = n_in, K = kin;
+  const float* Ws[2] = {W1, W2};
+  const float* bs[2] = {b1, b2};
+  float* hout[2] = {s_h1, s_h2};
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    const int c0 = 64 * wave;
+    if (c0 < n_hid) {
+      mf_f4 acc[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = mf_f4{0.f, 0.f, 0.f, 0.f};
+      clos_layer<4>(hin, lda, Ws[layer], kw, K, c0, lr, lk, acc);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int n = c0 + 16 * t + lr;
+        if (n < n_hid) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            hout[layer][(4 * lk + q) * LDH + n] = tanhf(acc[t][q] + bs[layer][n]);
+        }
+      }
+    }
+    __syncthreads();
+    hin = hout[layer];
+    lda = LDH;
+    kw = n_hid;
+    K = n_hid;
+  }
+  // output layer: wave w takes columns [16 w, 16 w + 16) (n_out <= 64)
+  const int c0 = 16 * wave;
+  if (c0 < n_out) {
+    mf_f4 acc[1] = {mf_f4{0.f, 0.f, 0.f, 0.f}};
+    clos_layer<1>(s_h2, LDH, W3, n_hid, n_hid, c0, lr, lk, acc);
+    const int n = c0 + lr;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = 4 * lk + q;
+      if (n < n_out && r0 + r < rows)
+        y[static_cast<long long>(r0 + r) * n_out + n] = acc[0][q] + b3[n];
+    }
   }
 }
 
@@ -2292,11 +2415,12 @@ int launch_hilb_mfma_n(const FormDev& d, const StageArgs& A, hipStream_t st) {
     throw HeomError{HEOM_ERR_UNSUPPORTED, "matrix-core path: too many noises for LDS"};
   const int grid = cap_grid(d.n_hrchy);
   const bool small = A.npre <= 4;
-  // FP64 and split-bf16 at N = 64: eight waves per ADO, each with half the
-  // output tiles (two workgroups per CU either way, so twice the waves work on
-  // the same ADOs; measured at 969 ADOs: 135 against 158 us per FP64
-  // Dormand-Prince stage, 99 against 138 us split-bf16; complex64 126 us both)
-  constexpr int BS = (N == 64 && (sizeof(R) == 8 || BF16X3)) ? 512 : kMfmaBS;
+  // N = 64: eight waves per ADO, each with half the output tiles (two
+  // workgroups per CU either way, so twice the waves work on the same ADOs;
+  // measured at 969 ADOs per Dormand-Prince stage: FP64 135 against 158 us,
+  // split-bf16 99 against 138 us, complex64 with the links outer 112 against
+  // 126 us)
+  constexpr int BS = N == 64 ? 512 : kMfmaBS;
 #define HAMD_MFMA(REALH, KP, PAD)                                                \
   {                                                                              \
     auto k = hilb_mfma_kernel<N, R, REALH, F64, KP, BF16X3, PAD, BS>;            \
@@ -2531,6 +2655,23 @@ void launch_closure_gather(const FormDev& d, const StageArgs& A, float* x, int a
   const long long n = static_cast<long long>(d.clos_n) * d.N2;
   hipLaunchKernelGGL(closure_gather_kernel, dim3(cap_grid((n + 255) / 256)), dim3(256), 0, stream,
                      d, A, x, ado_stride);
+  HAMD_HIP(hipGetLastError());
+}
+
+bool closure_mlp_fits(int n_in, int n_hid, int n_out) {
+  return n_in <= kClosInMax && n_hid % 16 == 0 && n_hid <= kClosHidMax && n_out % 16 == 0 &&
+         n_out <= 64;
+}
+
+void launch_closure_mlp(const FormDev& d, const StageArgs& A, const float* xc, int n_in,
+                        int n_hid, int n_out, const float* W1, const float* b1, const float* W2,
+                        const float* b2, const float* W3, const float* b3, float* y,
+                        hipStream_t stream) {
+  if (!closure_mlp_fits(n_in, n_hid, n_out))
+    throw HeomError{HEOM_ERR_UNSUPPORTED, "closure: network too wide for the fused kernel"};
+  const int grid = (d.clos_n + kClosRows - 1) / kClosRows;
+  hipLaunchKernelGGL(closure_mlp_kernel, dim3(grid), dim3(256), 0, stream, d, A, xc, n_in, n_hid,
+                     n_out, W1, b1, W2, b2, W3, b3, y);
   HAMD_HIP(hipGetLastError());
 }
 

@@ -84,6 +84,7 @@ struct Closure {
   int n_hid = 0;
   DevBuf W1, b1, W2, b2, W3, b3;  // folded copies
   DevBuf x, h1, h2, y;            // y: n_ado blocks of 2 N^2, added by the operator kernels
+  bool fused = false;             // one fused launch (launch_closure_mlp) per operator pass
   void bind(Qme& q, const Mlp& m, const float* features, int n_features, int layout);
   void forward(const Qme& q, const StageArgs& A, hipStream_t st);
 };

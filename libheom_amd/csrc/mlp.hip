@@ -628,10 +628,18 @@ void Closure::bind(Qme& q, const Mlp& m, const float* features, int n_features, 
   q.dev.clos = y.as<float>();
   q.dev.clos_lo = dm.lo;
   q.dev.clos_n = dm.n_ado;
+  // per-ADO rows of a network that fits: one fused launch per operator pass
+  fused = layout == HEOM_CLOSURE_PER_ADO && closure_mlp_fits(n_in, n_hid, n_out);
 }
 
 void Closure::forward(const Qme& q, const StageArgs& A, hipStream_t st) {
   const int n_in = dm.n_in, n_out = dm.n_out, rows = dm.rows;
+  if (fused) {
+    launch_closure_mlp(q.dev, A, x.as<float>(), n_in, n_hid, n_out, W1.as<float>(),
+                       b1.as<float>(), W2.as<float>(), b2.as<float>(), W3.as<float>(),
+                       b3.as<float>(), y.as<float>(), st);
+    return;
+  }
   launch_closure_gather(q.dev, A, x.as<float>(), dm.ado_stride, st);
   launch_sgemm(rows, n_hid, n_in, x.as<float>(), n_in, false, W1.as<float>(), n_in, true,
                h1.as<float>(), n_hid, b1.as<float>(), true, st);

@@ -2,6 +2,9 @@
 // members' host tables are packed into one set of device arrays so that one
 // operator launch per Runge-Kutta stage advances every member.
 #include <algorithm>
+#include <array>
+#include <cstdint>
+#include <map>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -27,7 +30,8 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
   kind = kBatch;
   batch_inst.clear();
   std::vector<int> lp{0};
-  std::vector<LinkFlat> fl;
+  std::vector<BatchLink> bl;
+  std::vector<BatchCoef> bc;
   std::vector<z128> gm, Hs, tab, dt;
   std::vector<BatchItem> items;
   long long elem = 0;
@@ -48,11 +52,28 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
     I.real_v = q.real_v ? 1 : 0;
     I.elem_base = elem;
     I.rs_off = rs;
+    I.coef_off = static_cast<int>(bc.size());
     rs += q.N2;
     batch_inst.push_back(I);
     const int base_link = lp.back();
     for (int l = 0; l < q.hs.n_hrchy; ++l) lp.push_back(base_link + q.link_ptr[l + 1]);
-    fl.insert(fl.end(), q.flat_host.begin(), q.flat_host.end());
+    {
+      // the member's distinct coefficient triples (compared bit for bit)
+      std::map<std::array<uint64_t, 7>, int> ids;
+      for (const LinkFlat& f : q.flat_host) {
+        std::array<uint64_t, 7> key{};
+        key[0] = static_cast<uint64_t>(f.u);
+        std::memcpy(&key[1], &f.iphi, sizeof(z128));
+        std::memcpy(&key[3], &f.psi, sizeof(z128));
+        std::memcpy(&key[5], &f.dir, sizeof(z128));
+        auto it = ids.find(key);
+        if (it == ids.end()) {
+          it = ids.emplace(key, static_cast<int>(ids.size())).first;
+          bc.push_back(BatchCoef{f.iphi, f.psi, f.dir});
+        }
+        bl.push_back(BatchLink{f.nbr, f.u, it->second, 0});
+      }
+    }
     gm.insert(gm.end(), q.ngamma.begin(), q.ngamma.end());
     Hs.insert(Hs.end(), q.H_host.begin(), q.H_host.end());
     tab.insert(tab.end(), q.cmac_host.begin(), q.cmac_host.end());
@@ -68,7 +89,8 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
   batch_rs_size = rs;
   hs = Hierarchy{};
   hs.n_hrchy = total_ado;
-  n_links = static_cast<long long>(fl.size());
+  n_links = static_cast<long long>(bl.size());
+  batch_n_coef = static_cast<long long>(bc.size());
   link_ptr = lp;
   links.clear();
   coef.clear();
@@ -76,7 +98,8 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
   real_v = false;
   HAMD_HIP(hipSetDevice(opt.device));
   d_link_ptr.upload(lp);
-  d_flat.upload(fl);
+  d_blink.upload(bl);
+  d_bcoef.upload(bc);
   d_gam.upload(gm);
   d_H.upload(Hs);
   d_cmac.upload(tab);
@@ -90,7 +113,8 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
   dev.n_hrchy = total_ado;
   dev.main_size = elem;
   dev.link_ptr = d_link_ptr.as<int>();
-  dev.flat = d_flat.as<LinkFlat>();
+  dev.blink = d_blink.as<BatchLink>();
+  dev.bcoef = d_bcoef.as<BatchCoef>();
   dev.gam = d_gam.as<z128>();
   dev.H = d_H.as<z128>();
   dev.cmac_tab = d_cmac.as<z128>();
@@ -111,7 +135,8 @@ void Qme::set_param_batch(const std::vector<Qme*>& members) {
 // stage of the whole batch (operator, two FP64 operands, two outputs: the
 // shape of an RK4 stage) for every tile shape -- work items per wavefront step
 // (1 or 2) and wavefronts per workgroup (1, 2 or 4) -- and grid size (4096 or
-// 16384 persistent workgroups, or one per item group), and keep the fastest.  The results do not depend on the choice beyond rounding
+// 16384 persistent workgroups, or one per item group), and keep the fastest
+// (one item per one-wavefront workgroup unless another shape is 3% faster).  The results do not depend on the choice beyond rounding
 // (the same expression per element).
 void Qme::autotune_batch() {
   const long long ms = main_size();
@@ -123,27 +148,38 @@ void Qme::autotune_batch() {
   acc.alloc(sizeof(z128) * ms);
   acc2.alloc(sizeof(z128) * ms);
   HAMD_HIP(hipMemsetAsync(a.p, 0, eb * ms, stream));
+  HAMD_HIP(hipMemsetAsync(b.p, 0, eb * ms, stream));
   HAMD_HIP(hipMemsetAsync(y.p, 0, sizeof(z128) * ms, stream));
   HAMD_HIP(hipMemsetAsync(acc.p, 0, sizeof(z128) * ms, stream));
-  StageArgs A;
-  std::memset(&A, 0, sizeof(A));
-  A.mode = MODE_STAGE;
-  A.in[0] = A.in[1] = a.p;
-  A.in_lo = lo();
-  A.a_re = 1e-3;
-  A.npre = 2;
-  for (int c = 0; c < 2; ++c) {
-    A.pre[c][0] = y.p;
-    A.pre[c][1] = acc.p;
-    A.dst[0][c] = b.p;
-    A.dst[1][c] = acc2.p;
+  HAMD_HIP(hipMemsetAsync(acc2.p, 0, sizeof(z128) * ms, stream));
+  // two stage passes that ping-pong their input / output and accumulator
+  // buffers, as consecutive Runge-Kutta stages do
+  StageArgs As[2];
+  for (int p = 0; p < 2; ++p) {
+    StageArgs& A = As[p];
+    std::memset(&A, 0, sizeof(A));
+    void* in = p ? b.p : a.p;
+    void* out = p ? a.p : b.p;
+    void* acc_in = p ? acc2.p : acc.p;
+    void* acc_out = p ? acc.p : acc2.p;
+    A.mode = MODE_STAGE;
+    A.in[0] = A.in[1] = in;
+    A.in_lo = lo();
+    A.a_re = 1e-3;
+    A.npre = 2;
+    for (int c = 0; c < 2; ++c) {
+      A.pre[c][0] = y.p;
+      A.pre[c][1] = acc_in;
+      A.dst[0][c] = out;
+      A.dst[1][c] = acc_out;
+    }
+    A.ncombo = 2;
+    A.cc[0][0] = 1.0;
+    A.ck[0] = 0.5;
+    A.cc[1][1] = 1.0;
+    A.ck[1] = 1.0 / 6.0;
+    A.dst_lo[0] = lo();
   }
-  A.ncombo = 2;
-  A.cc[0][0] = 1.0;
-  A.ck[0] = 0.5;
-  A.cc[1][1] = 1.0;
-  A.ck[1] = 1.0 / 6.0;
-  A.dst_lo[0] = lo();
   const int rb = lo() ? 4 : 8;
   hipEvent_t e0, e1;
   HAMD_HIP(hipEventCreate(&e0));
@@ -163,18 +199,19 @@ void Qme::autotune_batch() {
         dev.batch_grid = g;
         dev.batch_ipw = ipw;
         dev.batch_wpb = wpb;
-        launch_operator(dev, rb, A, stream);  // warm-up
+        launch_operator(dev, rb, As[0], stream);  // warm-up
+        launch_operator(dev, rb, As[1], stream);
         float shape_best = std::numeric_limits<float>::max();
-        for (int round = 0; round < 3; ++round) {  // best of three rounds of five passes
+        for (int round = 0; round < 3; ++round) {  // best of three rounds of six passes
           HAMD_HIP(hipEventRecord(e0, stream));
-          for (int r = 0; r < 5; ++r) launch_operator(dev, rb, A, stream);
+          for (int r = 0; r < 6; ++r) launch_operator(dev, rb, As[r & 1], stream);
           HAMD_HIP(hipEventRecord(e1, stream));
           HAMD_HIP(hipEventSynchronize(e1));
           float ms_ = 0.f;
           HAMD_HIP(hipEventElapsedTime(&ms_, e0, e1));
           shape_best = std::min(shape_best, ms_);
         }
-        tune_log.push_back(TuneEntry{ipw, wpb, g, 1e3f * shape_best / 5.0f});
+        tune_log.push_back(TuneEntry{ipw, wpb, g, 1e3f * shape_best / 6.0f});
         if (shape_best < best) {
           best = shape_best;
           best_g = g;
@@ -186,6 +223,17 @@ void Qme::autotune_batch() {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  // a shape other than one item per one-wavefront workgroup must be clearly
+  // faster (3%): closer timings are within what separates these passes from
+  // the solver's own stages
+  const int plain_g = std::min(dev.n_items, kMaxGrid);
+  for (const TuneEntry& t : tune_log) {
+    if (t.ipw == 1 && t.wpb == 1 && t.grid == plain_g && 1e3f * best / 6.0f > 0.97f * t.us) {
+      best_g = plain_g;
+      best_ipw = 1;
+      best_wpb = 1;
+    }
+  }
   dev.batch_grid = best_g;
   dev.batch_ipw = best_ipw;
   dev.batch_wpb = best_wpb;

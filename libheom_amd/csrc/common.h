@@ -161,7 +161,18 @@ struct BatchInst {
   int dterm_off;  // delta table (z128 units)
   int real_v;
   int rs_off;     // offset of the member's rho_S in a gathered all-members block
+  int coef_off;   // first entry of the member's link coefficients (BatchCoef)
   long long elem_base;  // first element of the member's state
+};
+
+// Link of the batch kernel: the neighbour, the noise index and the index of
+// its coefficients in the member's table (a member has few distinct ones: one
+// per mode, occupation and direction), 16 bytes instead of the 64 of LinkFlat.
+struct alignas(16) BatchLink {
+  int nbr, u, cid, pad;
+};
+struct alignas(16) BatchCoef {
+  z128 iphi, psi, dir;
 };
 
 // one wavefront of work: `apw` ADOs of member `inst` starting at local ADO `ado`
@@ -240,6 +251,8 @@ struct FormDev {
   int batch_wpb;  // wavefronts per workgroup (1, 2 or 4), picked by the autotuner
   const BatchInst* inst;
   const BatchItem* items;
+  const BatchLink* blink;  // links of all members (member order, as link_ptr)
+  const BatchCoef* bcoef;  // the members' coefficient tables
 };
 
 // Dormand-Prince 5(4) tableau (rkdp.h:21-57): A (stage inputs), B (5th-order

@@ -1378,6 +1378,7 @@ __device__ __forceinline__ void batch_load(const FormDev& d, const StageArgs& A,
   I.tab_off = insts[m].tab_off;
   I.dterm_off = insts[m].dterm_off;
   I.real_v = insts[m].real_v;
+  I.coef_off = insts[m].coef_off;
   I.elem_base = insts[m].elem_base;
   const int N = I.N, N2 = I.N2;
   const int la = lane / N2;
@@ -1418,9 +1419,10 @@ __device__ __forceinline__ void batch_compute(const FormDev& d, const StageArgs&
   cmac(D, conv<R>(d.gam[I.ado_base + b.ado]) + conv<R>(d.dterm[I.dterm_off + e]), b.r);
   const int l0 = d.link_ptr[I.ado_base + b.ado], l1 = d.link_ptr[I.ado_base + b.ado + 1];
   for (int l = l0; l < l1; ++l) {
-    const LinkFlat q = d.flat[l];
-    const C x = in[I.elem_base + static_cast<long long>(q.nbr) * N2 + e];
-    const int tb = q.u * N2 + e;
+    const BatchLink ql = d.blink[l];
+    const BatchCoef q = d.bcoef[I.coef_off + ql.cid];
+    const C x = in[I.elem_base + static_cast<long long>(ql.nbr) * N2 + e];
+    const int tb = ql.u * N2 + e;
     C cf;
     if (I.real_v) {
       const z128 p = d.cmac_tab[I.tab_off + tb];

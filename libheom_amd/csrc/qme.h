@@ -125,7 +125,8 @@ class Qme {
   std::vector<BatchInst> batch_inst;
   long long batch_main_size = 0;
   int batch_rs_size = 0;  // sum of the members' N^2
-  DevBuf d_inst, d_items;
+  DevBuf d_inst, d_items, d_blink, d_bcoef;
+  long long batch_n_coef = 0;  // entries of the members' coefficient tables
   DevBuf d_Hre, d_Him, d_Hre32, d_Him32;
   DevBuf d_Vre, d_Vim, d_Vre32, d_Vim32;  // dense V planes of the matrix-core path
   struct TuneEntry {

@@ -738,9 +738,11 @@ void Solver::stage_bytes(std::vector<double>& out) {
     // H, and V_u, Lambda_u, Lambda_u^+ per noise
     tables = 16.0 * q->N2 * (1.0 + 3.0 * static_cast<double>(q->noise.size()));
   } else {
-    const double lrec =
-        ((q->kind == HEOM_HILB || q->kind == kBatch) && q->diag_v) ? sizeof(LinkFlat) : sizeof(Link);
+    const double lrec = q->kind == kBatch                  ? sizeof(BatchLink)
+                        : (q->kind == HEOM_HILB && q->diag_v) ? sizeof(LinkFlat)
+                                                               : sizeof(Link);
     tables = lrec * q->n_links + 4.0 * (q->hs.n_hrchy + 1) + 16.0 * q->hs.n_hrchy;
+    if (q->kind == kBatch) tables += sizeof(BatchCoef) * static_cast<double>(q->batch_n_coef);
   }
   for (int s = 0; s < ns; ++s) {
     const StageSpec& a = specs[s];

@@ -102,9 +102,16 @@ def test_batch_tile_shapes_agree_and_match_the_oracle():
     print("autotuner:", chosen, tried, flush=True)
     assert {t[0] for t in tried} == {1, 2} and {t[1] for t in tried} == {1, 2, 4}
     assert all(t[3] > 0 for t in tried)
+    # the fastest shape, or one item per one-wavefront workgroup when nothing is
+    # 3% faster than that
     best = min(t[3] for t in tried)
-    assert (chosen["items_per_step"], chosen["waves_per_group"], chosen["grid"]) in \
-        [t[:3] for t in tried if t[3] == best]
+    plain = [t for t in tried if t[:2] == (1, 1) and t[2] == max(u[2] for u in tried
+                                                                 if u[:2] == (1, 1))][0]
+    shape = (chosen["items_per_step"], chosen["waves_per_group"], chosen["grid"])
+    if best > 0.97 * plain[3]:
+        assert shape == plain[:3]
+    else:
+        assert shape in [t[:3] for t in tried if t[3] == best]
     rng = np.random.default_rng(8)
     rho = rng.normal(size=b.main_size()) + 1j * rng.normal(size=b.main_size())
     outs = []

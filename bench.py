@@ -11,7 +11,10 @@ Metric: ADO-stage updates/s = n_hrchy * (RHS evaluations performed) / s.
 Also reported: the roofline of the dominant kernel (the fused Hilbert-form
 stage pass; algorithmic bytes per launch / average launch duration measured
 with HIP events on the library stream) and the CPU oracle timed on the host
-cores on a bounded sample of the same workload.
+cores on a bounded sample of the same workload.  The algorithmic bytes are
+those of the reference's step per ADO-stage (alg_bytes_per_ado_stage) times
+the ADOs of one launch; the bytes the fused kernel itself must move (fewer:
+stage 6 forms the error sum) are reported beside them.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -29,6 +32,18 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 GRAPHS = True  # replay stage sequences from HIP graphs (--no-graphs: direct launches)
+
+
+def alg_bytes_per_ado_stage(n_level, n_dim, method, sc=16):
+    """Algorithmic HBM bytes of one ADO-stage of the reference's stepping:
+    m N^2-blocks of sc bytes (sc = 16 complex128, 8 complex64), two int32
+    neighbour indices per mode and Gamma_n.  Dormand-Prince without FSAL
+    (rkdp.h:108-146): stage i of 1..6 reads y_i, y, k_1..k_{i-1} and writes k_i
+    and y_{i+1} (i + 3 blocks), stage 7 with the error norm moves 8: m = 47/7.
+    Fused RK4 (rk4.h:40-65): y_s, y, acc read, y_{s+1}, acc written: m = 5.
+    Neighbour ADOs count once (another ADO's compulsory read)."""
+    m = 47.0 / 7.0 if method == "rkdp" else 5.0
+    return m * n_level * n_level * sc + 8.0 * n_dim + 8.0
 
 
 def mk_solver(H, eng, q, method):
@@ -203,7 +218,11 @@ def bench_sweep(H, L, steps, warmup, reps):
     eng = H.hip(0)
     res = {}
     for dtype in ("complex64", "complex128"):
-        b = H.heom_batch([H.heom_hilb(m["max_depth"], dtype=dtype).load(m, eng) for m in ms])
+        qs = [H.heom_hilb(m["max_depth"], dtype=dtype).load(m, eng) for m in ms]
+        unit = sum(q.n_hrchy * alg_bytes_per_ado_stage(q.n_level, q.n_dim, "rk4",
+                                                       8 if dtype == "complex64" else 16)
+                   for q in qs)
+        b = H.heom_batch(qs)
         sv = mk_solver(H, eng, b, H.rk4())
         rho0 = b.pack([m["rho0"] for m in ms])
         nb = rho0.nbytes
@@ -229,7 +248,8 @@ def bench_sweep(H, L, steps, warmup, reps):
         _capi.check(L.heom_solver_time_stages(sv._s, reps, p(sms), ctypes.byref(ns)))
         _capi.check(L.heom_solver_stage_bytes(sv._s, p(sby), ctypes.byref(ns)))
         n = ns.value
-        gbs = float(sby[:n].mean()) / (float(sms[:n].mean()) * 1e-3) / 1e9
+        gbs = unit / (float(sms[:n].mean()) * 1e-3) / 1e9
+        gbs_kernel = float(sby[:n].mean()) / (float(sms[:n].mean()) * 1e-3) / 1e9
         final = np.zeros_like(rho0)
         _capi.check(L.heom_memcpy_d2h(p(final), d, nb))
         L.heom_dev_free(d)
@@ -238,7 +258,8 @@ def bench_sweep(H, L, steps, warmup, reps):
                       "autotune": dict(chosen, tried_us_per_pass={
                           "ipw%d_wpb%d_grid%d" % (k, w, g): round(u, 1) for k, w, g, u in tried}),
                       "stage_ms": float(sms[:n].mean()), "stage_bytes": float(sby[:n].mean()),
-                      "hbm_frac": gbs / HBM_PEAK_GBS, "rs": b.split_rho_s(rs),
+                      "hbm_frac": gbs / HBM_PEAK_GBS, "alg_bytes_per_stage": unit,
+                      "hbm_frac_kernel_bytes": gbs_kernel / HBM_PEAK_GBS, "rs": b.split_rho_s(rs),
                       "final": b.unpack(final),
                       "n_hrchy": b.n_hrchy, "main_size": b.main_size()}
     # members whose FP64 rho_S stays a bounded density matrix (|rho_ij| <= 1);
@@ -275,9 +296,12 @@ def bench_sweep(H, L, steps, warmup, reps):
         "n_hrchy_total": f64["n_hrchy"], "main_size": f64["main_size"],
         "dtype": "f32 operator / f64 combination", "value": f32["value"],
         "ms_per_step": f32["ms_per_step"], "stage_ms": f32["stage_ms"],
-        "hbm_frac": f32["hbm_frac"], "autotune": f32["autotune"],
+        "hbm_frac": f32["hbm_frac"], "hbm_frac_kernel_bytes": f32["hbm_frac_kernel_bytes"],
+        "alg_bytes_per_stage": f32["alg_bytes_per_stage"], "autotune": f32["autotune"],
         "fp64": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
                  "stage_ms": f64["stage_ms"], "hbm_frac": f64["hbm_frac"],
+                 "hbm_frac_kernel_bytes": f64["hbm_frac_kernel_bytes"],
+                 "alg_bytes_per_stage": f64["alg_bytes_per_stage"],
                  "autotune": f64["autotune"]},
         "fp32_batch_speedup_vs_fp64_batch": f64["ms_per_step"] / f32["ms_per_step"],
         "rho_s_max_abs_err_vs_fp64_bounded_members": worst,
@@ -354,8 +378,13 @@ def bench_dense64(H, L, reps):
         # real flops per ADO-stage of the commutator (two complex N^3 products; H real
         # symmetric here, so 4 real N^3 GEMMs) -- the MFMA-relevant part
         flops = q.n_hrchy * 4 * 2 * 64 ** 3
+        unit = q.n_hrchy * alg_bytes_per_ado_stage(
+            q.n_level, q.n_dim, "rkdp", 16 if dtype in ("complex128", "valu") else 8)
         res[dtype] = {"value": st["ado_stage_updates"] / el, "time_to_rho_s_ms": 1e3 * el,
-                      "stage_ms": stage_ms, "hbm_frac": float(sby[:n].mean()) / (stage_ms * 1e-3)
+                      "stage_ms": stage_ms,
+                      "hbm_frac": unit / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "alg_bytes_per_stage": unit,
+                      "hbm_frac_kernel_bytes": float(sby[:n].mean()) / (stage_ms * 1e-3)
                       / 1e9 / HBM_PEAK_GBS,
                       "commutator_tflops": flops / (stage_ms * 1e-3) / 1e12,
                       "rhs_evals": st["n_rhs"], "rs": rs}
@@ -685,8 +714,10 @@ def main():
     _capi.check(L.heom_solver_time_stages(solver._s, args.reps, p(ms), ctypes.byref(ns)))
     _capi.check(L.heom_solver_stage_bytes(solver._s, p(by), ctypes.byref(ns)))
     n = ns.value
-    avg_bytes = float(by[:n].mean())
+    kernel_bytes = float(by[:n].mean())
     avg_ms = float(ms[:n].mean())
+    unit_bytes = alg_bytes_per_ado_stage(qme.n_level, qme.n_dim, "rkdp")
+    avg_bytes = unit_bytes * qme.n_hrchy
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
     L.heom_dev_free(d)
     # measured HBM bytes per stage dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
@@ -713,8 +744,11 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "hilb_fast_kernel<7,double> (fused Dormand-Prince stage pass)",
+            "bytes_per_ado_stage": unit_bytes, "ados_per_launch": qme.n_hrchy,
             "bytes_per_launch": avg_bytes, "avg_launch_ms": avg_ms,
             "stage_ms": [float(x) for x in ms[:n]],
+            "kernel_bytes_per_launch": kernel_bytes,
+            "frac_kernel_bytes": kernel_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "rho_s_max_abs_err_vs_oracle": (float(np.abs(rs - gold).max())
                                         if gold is not None else None),

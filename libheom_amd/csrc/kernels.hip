@@ -45,6 +45,21 @@ __device__ __forceinline__ LinkFlat ld_flat(cptr<LinkFlat> p) {
   return f;
 }
 
+// Link l from the 8-byte record (neighbour, coefficient index) and its entry in
+// the hierarchy's deduplicated coefficient table (one entry per mode,
+// occupation and direction: a few KB, held by the scalar cache); i*phi is
+// formed exactly (a swap and a negation), as the host does for LinkFlat.
+__device__ __forceinline__ LinkFlat ld_link_coef(cptr<Link> lk, cptr<Coef> cf, int l) {
+  LinkFlat f;
+  const int c = lk[l].cid;
+  f.nbr = lk[l].nbr;
+  f.u = cf[c].u;
+  f.iphi = z128{-cf[c].phi.im, cf[c].phi.re};
+  f.psi = z128{cf[c].psi.re, cf[c].psi.im};
+  f.dir = z128{cf[c].dir.re, cf[c].dir.im};
+  return f;
+}
+
 __device__ __forceinline__ z128 ldz(const void* p, bool lo, long long i) {
   if (!lo) return static_cast<const z128*>(p)[i];
   const z64 v = static_cast<const z64*>(p)[i];
@@ -282,7 +297,8 @@ __device__ __forceinline__ void add_closure(const FormDev& d, long long gi, cx<R
 // Workgroup b takes the ADO of work unit bmap[b] (its link range in the same
 // record), read beside the control block.  H and the diagonals of every V_u
 // are staged in LDS once per ADO (no per-link coefficient tables), each link is
-// one 64-byte scalar record load and one neighbour-element load.
+// an 8-byte scalar record, its coefficient-table entry and one neighbour-element
+// load (the 64-byte LinkFlat records cost 8% more time per stage).
 // REAL: H and every V_u real (the commutator and the link coefficients in real
 // arithmetic on the real parts; a complex H or V takes REAL = false).
 // ---------------------------------------------------------------------------
@@ -340,7 +356,8 @@ __global__ void __launch_bounds__(WaveCfg<N>::BS, (KP > 4 && WaveCfg<N>::BS == 6
   const int ee = lane_ok ? e : 0;
   const int i = ee / N, j = ee - (ee / N) * N;
   const C* in = static_cast<const C*>(pick(A.in, cur));
-  const auto flat = cst(d.flat);
+  const auto lks = cst(d.links);
+  const auto cfs = cst(d.coef);
 
   // idle lanes (e >= N2) read element 0 and store nothing
   const int g = bm.g;
@@ -359,7 +376,7 @@ __global__ void __launch_bounds__(WaveCfg<N>::BS, (KP > 4 && WaveCfg<N>::BS == 6
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     if (t < nb0) {
-      q[t] = ld_flat(flat + l0 + t);
+      q[t] = ld_link_coef(lks, cfs, l0 + t);
       x[t] = in[static_cast<long long>(q[t].nbr) * N2 + ee];
     }
   }
@@ -424,7 +441,7 @@ __global__ void __launch_bounds__(WaveCfg<N>::BS, (KP > 4 && WaveCfg<N>::BS == 6
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       if (t < nb) {
-        q[t] = ld_flat(flat + l + t);
+        q[t] = ld_link_coef(lks, cfs, l + t);
         x[t] = in[static_cast<long long>(q[t].nbr) * N2 + ee];
       }
     }

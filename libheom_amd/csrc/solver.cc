@@ -738,11 +738,16 @@ void Solver::stage_bytes(std::vector<double>& out) {
     // H, and V_u, Lambda_u, Lambda_u^+ per noise
     tables = 16.0 * q->N2 * (1.0 + 3.0 * static_cast<double>(q->noise.size()));
   } else {
+    // the fast Hilbert kernel (diagonal V, 6 <= N <= 16) reads 8-byte links and
+    // the coefficient table, the other diagonal-V Hilbert kernels LinkFlat
+    const bool fast = q->kind == HEOM_HILB && q->diag_v && q->N >= 6 && q->N <= 16;
     const double lrec = q->kind == kBatch                  ? sizeof(BatchLink)
+                        : fast                              ? sizeof(Link)
                         : (q->kind == HEOM_HILB && q->diag_v) ? sizeof(LinkFlat)
                                                                : sizeof(Link);
     tables = lrec * q->n_links + 4.0 * (q->hs.n_hrchy + 1) + 16.0 * q->hs.n_hrchy;
     if (q->kind == kBatch) tables += sizeof(BatchCoef) * static_cast<double>(q->batch_n_coef);
+    if (fast) tables += sizeof(Coef) * static_cast<double>(q->coef.size());
   }
   for (int s = 0; s < ns; ++s) {
     const StageSpec& a = specs[s];

@@ -123,9 +123,6 @@ int heom_create(int kind, int dtype, int n_level, int max_depth, const heom_opti
     if (opt) o = *opt;
     if (o.order != 0 && o.order != 1)
       throw hamd::HeomError{HEOM_ERR_ARG, "heom_create: order must be 0 or 1"};
-    if (o.order == 1 && kind != HEOM_HILB && !redfield)
-      throw hamd::HeomError{HEOM_ERR_UNSUPPORTED,
-                            "col_major is provided for heom_hilb and the Redfield forms"};
     auto h = std::make_unique<heom_qme>();
     h->q = std::make_shared<hamd::Qme>(kind, dtype, n_level, max_depth, o);
     *out = h.release();

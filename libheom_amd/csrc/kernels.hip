@@ -1724,6 +1724,111 @@ __global__ void __launch_bounds__(256) liou_kernel(const FormDev d, const StageA
   if (A.do_err) write_partial(A, err_acc);
 }
 
+// heom_liou for 16 < N <= 32 (N^2 up to 1024): one ADO per 256-lane workgroup,
+// each lane up to four elements e = tid + 256 c, rho_n and the per-noise
+// neighbour sums phi, psi staged in LDS (3 N^2 complex values, dynamic);
+// the same per-element operation order as liou_kernel.
+template <typename R, bool F64, int KP>
+__global__ void __launch_bounds__(256) liou_wide_kernel(const FormDev d, const StageArgs A) {
+  using C = cx<R>;
+  constexpr int NC = 4;
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  const int N2 = d.N2;
+  C* s_rho = reinterpret_cast<C*>(s_dyn);
+  C* s_phi = s_rho + N2;
+  C* s_psi = s_phi + N2;
+  const Prologue pg = prologue(A);
+  if (pg.skip) return;
+  const int cur = pg.cur;
+  const int tid = threadIdx.x;
+  const C* in = static_cast<const C*>(pick(A.in, cur));
+  const int stride = N2 + 1;
+  double err_acc = 0.0;
+  for (int n = blockIdx.x; n < d.n_hrchy; n += gridDim.x) {
+    const long long base = static_cast<long long>(n) * N2;
+    for (int e = tid; e < N2; e += 256) s_rho[e] = in[base + e];
+    __syncthreads();
+    C D[NC];
+    const C gm = conv<R>(d.gam[n]);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int e = tid + 256 * c;
+      D[c] = C{0, 0};
+      if (e < N2) {
+        for (int p = d.sop_ptr[e]; p < d.sop_ptr[e + 1]; ++p)
+          cmac(D[c], conv<R>(d.sop_val[p]), s_rho[d.sop_col[p]]);
+        cmac(D[c], gm, s_rho[e]);
+      }
+    }
+    const int l1 = d.link_ptr[n + 1];
+    int l = d.link_ptr[n];
+    for (int u = 0; u < d.n_noise; ++u) {
+      C phi[NC], psi[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) phi[c] = psi[c] = C{0, 0};
+      while (l < l1) {
+        const Link lk = d.links[l];
+        const Coef cf = d.coef[lk.cid];
+        if (cf.u != u) break;
+        const C* nb = in + static_cast<long long>(lk.nbr) * N2;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const int e = tid + 256 * c;
+          if (e < N2) {
+            const C x = nb[e];
+            cmac(phi[c], conv<R>(cf.phi), x);
+            cmac(psi[c], conv<R>(cf.psi), x);
+          }
+        }
+        ++l;
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int e = tid + 256 * c;
+        if (e < N2) {
+          s_phi[e] = phi[c];
+          s_psi[e] = psi[c];
+        }
+      }
+      __syncthreads();
+      const int* pp = d.sop_ptr + (1 + 2 * u) * stride;
+      const int* pq = d.sop_ptr + (2 + 2 * u) * stride;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int e = tid + 256 * c;
+        if (e < N2) {
+          for (int p = pp[e]; p < pp[e + 1]; ++p)
+            cmac(D[c], conv<R>(d.sop_val[p]), s_phi[d.sop_col[p]]);
+          for (int p = pq[e]; p < pq[e + 1]; ++p)
+            cmac(D[c], conv<R>(d.sop_val[p]), s_psi[d.sop_col[p]]);
+        }
+      }
+      __syncthreads();
+    }
+    for (; l < l1; ++l) {
+      const Link lk = d.links[l];
+      const Coef cf = d.coef[lk.cid];
+      const C* nb = in + static_cast<long long>(lk.nbr) * N2;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int e = tid + 256 * c;
+        if (e < N2) cmac(D[c], conv<R>(cf.dir), nb[e]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int e = tid + 256 * c;
+      if (e < N2) {
+        PreT<KP> pv;
+        prefetch<F64, KP>(pv, A, cur, base + e, true);
+        err_acc += epilogue<F64, R, KP>(pv, A, cur, base + e, D[c], pg.alpha);
+      }
+    }
+    __syncthreads();
+  }
+  if (A.do_err) write_partial(A, err_acc);
+}
+
 // ---------------------------------------------------------------------------
 // heom_ado: one global CSR matrix-vector product (heom_ado.h:191-202)
 // ---------------------------------------------------------------------------
@@ -2624,6 +2729,17 @@ int launch_form(const FormDev& d, const StageArgs& A, hipStream_t st) {
 #undef HAMD_BATCH_W
 #undef HAMD_BATCH
     return grid * wpb;  // error-norm partials: one per wavefront
+  }
+  if (d.kind == 1 && d.N2 > 256) {
+    // 16 < N <= 32: one ADO per workgroup (set_param refuses larger N)
+    const int grid = cap_grid(d.n_hrchy);
+    const size_t lds = 3 * sizeof(cx<R>) * static_cast<size_t>(d.N2);
+    if (A.npre <= 4)
+      hipLaunchKernelGGL((liou_wide_kernel<R, F64, 4>), dim3(grid), dim3(256), lds, st, d, A);
+    else
+      hipLaunchKernelGGL((liou_wide_kernel<R, F64, kMaxPre>), dim3(grid), dim3(256), lds, st, d,
+                         A);
+    return grid;
   }
   if (d.kind == 1) {
     const int APB = 256 / d.N2;

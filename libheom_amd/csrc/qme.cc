@@ -249,7 +249,7 @@ void Qme::set_param() {
   std::vector<int> sop_ptr, sop_col;
   std::vector<z128> sop_val;
   if (kind == HEOM_LIOU || kind == HEOM_ADO) {
-    if (N > 16) throw HeomError{HEOM_ERR_UNSUPPORTED, "heom_liou/heom_ado: n_level > 16"};
+    if (N > 32) throw HeomError{HEOM_ERR_UNSUPPORTED, "heom_liou/heom_ado: n_level > 32"};
     build_superops(sop_ptr, sop_col, sop_val, ops);
   }
   if (kind == HEOM_ADO) build_ado(ops);

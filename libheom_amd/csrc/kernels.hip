@@ -1435,8 +1435,12 @@ __device__ __forceinline__ void batch_compute(const FormDev& d, const StageArgs&
   C D = times_i(acc);
   cmac(D, conv<R>(d.gam[I.ado_base + b.ado]) + conv<R>(d.dterm[I.dterm_off + e]), b.r);
   const int l0 = d.link_ptr[I.ado_base + b.ado], l1 = d.link_ptr[I.ado_base + b.ado + 1];
+  // the next link record is loaded with the current link's coefficient,
+  // neighbour and table loads (one memory round trip per link instead of two)
+  BatchLink qn = l0 < l1 ? d.blink[l0] : BatchLink{0, 0, 0, 0};
   for (int l = l0; l < l1; ++l) {
-    const BatchLink ql = d.blink[l];
+    const BatchLink ql = qn;
+    if (l + 1 < l1) qn = d.blink[l + 1];
     const BatchCoef q = d.bcoef[I.coef_off + ql.cid];
     const C x = in[I.elem_base + static_cast<long long>(ql.nbr) * N2 + e];
     const int tb = ql.u * N2 + e;

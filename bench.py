@@ -515,11 +515,12 @@ def bench_redfield(H, L, reps, cpu=True):
     return out
 
 
-def profile_stages(H, name, reps):
+def profile_stages(H, name, reps, dtype="complex128"):
     """One configuration's stage kernels only (for rocprofv3 counter passes):
     the 7-site chain (Dormand-Prince, 12650 ADOs), the N=64 dense config
-    (Dormand-Prince, 969 ADOs), FP64, or the 512-member sweep (RK4, one batch,
-    complex64 operator then FP64), every kernel launched directly."""
+    (Dormand-Prince, 969 ADOs), in `dtype` (complex128, complex64 or
+    complex64_bf16x3), or the 512-member sweep (RK4, one batch, complex64
+    operator then FP64), every kernel launched directly."""
     from libheom_amd import _capi, configs
     L = _capi.load()
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
@@ -542,7 +543,7 @@ def profile_stages(H, name, reps):
                               "stage_bytes": [float(x) for x in sby[:n]]}))
         return
     m = configs.exciton_chain7(t_final=20.0) if name == "7site" else configs.dense64(t_final=0.2)
-    q = H.heom_hilb(m["max_depth"]).load(m, eng)
+    q = H.heom_hilb(m["max_depth"], dtype=dtype).load(m, eng)
     sv = H.qme_solver(eng, q, H.rkdp())
     sv.set_graphs(False)
     sv.solve_rho_s(m["rho0"], m["t_list"][:2], {"dt": m["dt"], "atol": m["atol"], "rtol": m["rtol"]})
@@ -574,6 +575,9 @@ def main():
                     help="counter runs: set up one configuration, run one short solve and "
                          "--reps timed passes of every stage kernel, print the stage times "
                          "and exit")
+    ap.add_argument("--profile-dtype", default="complex128",
+                    choices=["complex128", "complex64", "complex64_bf16x3"],
+                    help="state type of --profile-config 7site / dense64")
     ap.add_argument("--summarize-trace", metavar="DB",
                     help="print a table of a rocprofv3 --kernel-trace run (*_results.db) and "
                          "exit")
@@ -613,7 +617,7 @@ def main():
     from libheom_amd import _capi, configs
 
     if args.profile_config:
-        profile_stages(H, args.profile_config, args.reps)
+        profile_stages(H, args.profile_config, args.reps, args.profile_dtype)
         return
 
     m = configs.exciton_chain7(t_final=args.t_final)

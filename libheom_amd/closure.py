@@ -193,3 +193,92 @@ def generate(deep_models, shallow_models, dt, t_list, features, engine, chunk=25
         if progress:
             progress(idx[-1] + 1, n_models)
     return Rows(X, Y, row, rho_s, t_list)
+
+
+def rollout_rows(deep_models, shallow_models, dt, t_list, features, engine, net, chunk=250,
+                 layout=_capi.HEOM_CLOSURE_PER_ADO, bound=1e6, progress=None):
+    """Closure rows along closure-driven runs (data seen by the closure itself).
+
+    Each shallow model is propagated (RK4) with `net` bound as its closure,
+    beside its deep run.  At every t in t_list[1:] its rows take the inputs
+    from the closure-driven run's last tier and the targets (the tier-(L+1)
+    term truncation drops) from the deep state at the same t.  The shallow
+    hierarchy is the first ADOs of the deep one (both enumerated breadth-first,
+    tier by tier), so the rows are built from the deep state with its first
+    ADOs replaced by the closure-driven state.  A run whose state leaves
+    |rho| <= bound (or turns non-finite) gives no further rows.
+
+    Returns (X, Y, n_rows) with X, Y on the device."""
+    from . import heom_batch, heom_hilb, qme_solver, rk4
+    L = _capi.load()
+    n_models = len(deep_models)
+    t_list = np.asarray(t_list, np.float64)
+    per_model, n_in, n_out = None, 0, 0
+    X = Y = None
+    row = 0
+    for c0 in range(0, n_models, chunk):
+        idx = list(range(c0, min(n_models, c0 + chunk)))
+        deep = [heom_hilb(deep_models[i]["max_depth"]).load(deep_models[i], engine) for i in idx]
+        shallow = [heom_hilb(shallow_models[i]["max_depth"]).load(shallow_models[i], engine)
+                   for i in idx]
+        if per_model is None:
+            per_model, n_in, n_out = dims(shallow[0], layout, len(features[0]))
+            total = n_models * (t_list.size - 1) * per_model
+            X = DevArray(total * n_in * 4)
+            Y = DevArray(total * n_out * 4)
+        # closure-driven runs: own QMEs with the network bound, device states
+        runs = []
+        for k, i in enumerate(idx):
+            q = heom_hilb(shallow_models[i]["max_depth"]).load(shallow_models[i], engine)
+            q.set_closure(net, features[i], layout)
+            sv = qme_solver(engine, q, rk4())
+            st = DevArray.from_host(np.ascontiguousarray(shallow_models[i]["rho0"],
+                                                         np.complex128))
+            runs.append([q, sv, st, True])
+        b = heom_batch(deep)
+        sv_d = qme_solver(engine, b, rk4())
+        state = DevArray.from_host(b.pack([deep_models[i]["rho0"] for i in idx]))
+        n_deep = max(q.main_size() for q in deep)
+        mixed = DevArray(n_deep * 16)
+        for j in range(1, t_list.size):
+            tt = np.ascontiguousarray(t_list[j - 1:j + 1])
+            _capi.check(L.heom_solve_dev(sv_d._s, state.ptr, _p(tt), 2, dt, 0.0, 0.0, None))
+            for k, i in enumerate(idx):
+                q, sv, st, live = runs[k]
+                if not live:
+                    continue
+                _capi.check(L.heom_solve_dev(sv._s, st.ptr, _p(tt), 2, dt, 0.0, 0.0, None))
+                ns = q.main_size()
+                rs = st.to_host(np.complex128, (ns,))
+                if not np.isfinite(rs).all() or np.abs(rs).max() > bound:
+                    runs[k][3] = False
+                    continue
+                nd = deep[k].main_size()
+                full = state.to_host(np.complex128, (nd,), int(b.elem_base[k]) * 16)
+                full[:ns] = rs
+                _capi.check(L.heom_memcpy_h2d(mixed.ptr, _p(full), full.nbytes))
+                f = np.ascontiguousarray(features[i], np.float32)
+                _capi.check(L.heom_closure_rows(
+                    deep[k]._h, shallow[k]._h, mixed.ptr, _p(f), f.size, layout,
+                    X.addr(row * n_in * 4), Y.addr(row * n_out * 4)))
+                row += per_model
+        del sv_d, b, state, runs
+        if progress:
+            progress(idx[-1] + 1, n_models)
+    return X, Y, row
+
+
+def concat_rows(parts, n_in, n_out):
+    """One device row set from several (X, Y, n_rows) parts (through the host)."""
+    n = sum(p[2] for p in parts)
+    X, Y = DevArray(n * n_in * 4), DevArray(n * n_out * 4)
+    r = 0
+    for px, py, m in parts:
+        if m == 0:
+            continue
+        hx = px.to_host(np.float32, (m, n_in))
+        hy = py.to_host(np.float32, (m, n_out))
+        _capi.check(_capi.load().heom_memcpy_h2d(X.addr(r * n_in * 4), _p(hx), hx.nbytes))
+        _capi.check(_capi.load().heom_memcpy_h2d(Y.addr(r * n_out * 4), _p(hy), hy.nbytes))
+        r += m
+    return X, Y, n

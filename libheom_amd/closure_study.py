@@ -12,6 +12,11 @@ models seeds 10000 .. 10000+held_out-1.
    last tier): input (Re, Im of rho_n, n, model features), target the
    tier-4 term that depth-3 truncation drops.
 2. A 3-layer width-256 tanh network is trained on them (Adam, batch 4096).
+2b. Rollout rounds (--rollout-rounds, default 2): the first --rollout-models
+   training models are run at depth 3 with the network as their closure; the
+   rows of those closure-driven states (inputs from the closure-driven last
+   tier, targets from the deep run at the same t) join the training rows and
+   the network is trained on the union for --rollout-epochs more epochs.
 3. Gain: the network's output scaled by g in {0, 0.25, 0.5, 0.75, 1} on 20
    validation models (seeds 20000 ..), keeping the g with the lowest median
    rho_S error against their depth-8 runs (g = 0 is plain truncation).
@@ -36,6 +41,9 @@ def main(argv=None):
     ap.add_argument("--held-out", type=int, default=100)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--deep", type=int, default=8)
+    ap.add_argument("--rollout-rounds", type=int, default=2)
+    ap.add_argument("--rollout-models", type=int, default=500)
+    ap.add_argument("--rollout-epochs", type=int, default=10)
     a = ap.parse_args(argv)
 
     import libheom_amd as H
@@ -71,6 +79,26 @@ def main(argv=None):
                             lr=1e-3, seed=2 + e0).tolist()
         log("epoch %d loss %.4f" % (len(losses), losses[-1]))
     t_train = time.time() - tt
+    # rollout rounds: rows of the closure-driven states join the training set
+    parts = [(d_tr.X, d_tr.Y, d_tr.n_rows)]
+    ro = {"rounds": a.rollout_rounds, "models": a.rollout_models, "rows": [], "loss": []}
+    tr = time.time()
+    ro_seeds = list(range(min(a.rollout_models, a.train)))
+    for r in range(a.rollout_rounds):
+        md = [configs.chain4(s, max_depth=a.deep) for s in ro_seeds]
+        ms = [configs.chain4(s, max_depth=L) for s in ro_seeds]
+        fr = [C.chain_features(m) for m in ms]
+        xr, yr, nr = C.rollout_rows(md, ms, dt, t_list, fr, eng, net, chunk=250,
+                                    progress=lambda i, n: log("rollouts %d / %d" % (i, n)))
+        parts.append((xr, yr, nr))
+        ro["rows"].append(nr)
+        Xu, Yu, nu = C.concat_rows(parts, 49, 32)
+        lr_ = net.train(Xu, Yu, nu, batch=4096, epochs=a.rollout_epochs, lr=3e-4,
+                        seed=100 + r).tolist()
+        ro["loss"].append([lr_[0], lr_[-1]])
+        log("rollout round %d: %d rows, loss %.4f" % (r + 1, nr, lr_[-1]))
+        del Xu, Yu
+    ro["seconds"] = time.time() - tr
     X = d_te.X.to_host(np.float32, (d_te.n_rows, 49))
     Y = d_te.Y.to_host(np.float32, (d_te.n_rows, 32)).astype(np.float64)
     P = net.forward(X).astype(np.float64)
@@ -123,6 +151,7 @@ def main(argv=None):
                     "runs (RK4 dt=0.01, t=0..10, 20 states each)" % (a.train, a.deep),
         "training_rows": d_tr.n_rows, "deep_runs_s": t_gen, "training_s": t_train,
         "epochs": a.epochs, "loss_first_last": [losses[0], losses[-1]],
+        "rollout_training": ro,
         "held_out_models": a.held_out,
         "held_out_fraction_of_dropped_term_recovered": rec,
         "rho_s_error_vs_deep": {k: {"median": float(np.median(v)), "mean": float(np.mean(v)),

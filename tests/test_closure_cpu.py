@@ -53,3 +53,19 @@ def test_column_scaling_constant_column():
     X = np.array([[1.0, 3.0], [2.0, 3.0], [3.0, 3.0]])
     mu, sd = MO.column_scaling(X)
     assert np.allclose(mu, [2.0, 3.0]) and np.allclose(sd, [np.sqrt(2.0 / 3.0), 1.0])
+
+
+def test_shallow_hierarchy_is_the_prefix_of_the_deep_one():
+    """closure.rollout_rows builds a deep-sized state whose first ADOs are the
+    closure-driven depth-L state: the depth-L hierarchy must be the first
+    C(K+L, L) ADOs of the deep one, in the same order (breadth-first, tier by
+    tier, hrchy_space.cc:160-229), and the next ADO must start tier L+1."""
+    from libheom_amd import configs
+    from oracle import oracle as O
+    for K, L, deep in [(8, 3, 8), (2, 4, 9), (5, 1, 4), (3, 3, 5)]:
+        m = configs.random_model(2, K, 1, deep, seed=K, dense_v=False)
+        n_deep = O.Oracle(O.HILB, 2, deep).load(m).tables()[0]
+        m["max_depth"] = L
+        n_sh = O.Oracle(O.HILB, 2, L).load(m).tables()[0]
+        assert np.array_equal(n_deep[:n_sh.shape[0]], n_sh), (K, L, deep)
+        assert n_sh.sum(1).max() == L and n_deep[n_sh.shape[0]].sum() == L + 1

@@ -227,3 +227,39 @@ def test_trained_closure_recovers_the_dropped_term():
           % (losses[0], losses[-1], recovered))
     assert losses[-1] < 0.5 * losses[0]
     assert recovered > 0.15
+
+
+def test_rollout_rows_pair_closure_driven_inputs_with_deep_targets():
+    """closure.rollout_rows (training on the closure's own runs): the targets
+    are the deep run's dropped term (the deep-state rows' targets, to the FP64
+    rounding of the deep-minus-truncated derivative difference, whose tier-3
+    part now comes from another state), the constant columns the same, and the
+    Re/Im columns the last tier of the closure-driven depth-3 run at that time."""
+    L, Ld, dt = 3, 5, 0.01
+    t_list = np.arange(4) * 0.5
+    seeds = list(range(6))
+    mk = lambda d: [configs.chain4(s, max_depth=d) for s in seeds]
+    f = [C.chain_features(m) for m in mk(L)]
+    d = C.generate(mk(Ld), mk(L), dt, t_list, f, eng(), chunk=4)
+    net = C.Mlp(49, 256, 32, seed=1)
+    net.fit_scaling(d.X, d.Y, d.n_rows)
+    xr, yr, nr = C.rollout_rows(mk(Ld), mk(L), dt, t_list, f, eng(), net, chunk=4)
+    assert nr == d.n_rows
+    Xg = d.X.to_host(np.float32, (nr, 49))
+    Yg = d.Y.to_host(np.float32, (nr, 32))
+    Xr = xr.to_host(np.float32, (nr, 49))
+    Yr = yr.to_host(np.float32, (nr, 32))
+    err = float(np.abs(Yr.astype(np.float64) - Yg).max())
+    print("rollout targets vs deep-state targets: max abs difference %.2e" % err)
+    assert err <= 1e-10
+    assert np.array_equal(Xr[:, 32:], Xg[:, 32:])
+    assert not np.array_equal(Xr[:, :32], Xg[:, :32])  # the closure moved the run
+    # model 0 at t = 0.5: its rows are the first 120; rerun it with the closure
+    m = configs.chain4(0, max_depth=L)
+    q = H.heom_hilb(L).load(m, eng())
+    q.set_closure(net, f[0])
+    rho, _ = H.qme_solver(eng(), q, H.rk4()).solve_rho_s(m["rho0"], t_list[:2], {"dt": dt})
+    last = rho.reshape(-1, 16)[45:]
+    assert last.shape[0] == 120
+    assert np.array_equal(Xr[:120, :16], last.real.astype(np.float32))
+    assert np.array_equal(Xr[:120, 16:32], last.imag.astype(np.float32))

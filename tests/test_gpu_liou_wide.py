@@ -90,6 +90,21 @@ def test_col_major_is_transposed_storage(form, N, dense_v, sq):
     assert np.abs(got.transpose(0, 2, 1) - ref).max() <= TOL * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("form", ["liou", "ado"])
+def test_wide_complex64_calc_diff(form):
+    """complex64 state and operator (FP32 arithmetic): error against the FP64
+    oracle stated, bar 1e-5 relative as for the other complex64 forms."""
+    cls, okind = FORMS[form]
+    m = configs.random_model(24, 2, 1, 2, seed=324, dense_v=True, offdiag_gamma=True)
+    g = cls(m["max_depth"], dtype="complex64").load(m, eng())
+    o = O.Oracle(okind, m["N"], m["max_depth"], n_threads=8).load(m)
+    ref = o.calc_diff(m["rho0"], 1.0, 0.0)
+    got = g.calc_diff_impl(None, np.zeros_like(ref), m["rho0"], 1.0, 0.0)
+    err = np.abs(got - ref).max() / max(1.0, np.abs(ref).max())
+    print("wide complex64", form, "relative error vs FP64 oracle %.2e" % err)
+    assert err <= 1e-5
+
+
 def test_liou_beyond_n_level_32_is_refused():
     m = configs.random_model(33, 1, 1, 1, seed=1, dense_v=False)
     with pytest.raises(H.HeomError) as e:

@@ -66,6 +66,18 @@ __device__ __forceinline__ z128 ldz(const void* p, bool lo, long long i) {
   return z128{static_cast<double>(v.re), static_cast<double>(v.im)};
 }
 
+// ldz with a non-temporal load (streamed operands that later loads must not evict)
+__device__ __forceinline__ z128 ldz_nt(const void* p, bool lo, long long i) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  if (!lo) {
+    const d2v w = __builtin_nontemporal_load(static_cast<const d2v*>(p) + i);
+    return z128{w.x, w.y};
+  }
+  const f2v w = __builtin_nontemporal_load(static_cast<const f2v*>(p) + i);
+  return z128{static_cast<double>(w.x), static_cast<double>(w.y)};
+}
+
 __device__ __forceinline__ void stz(void* p, bool lo, long long i, z128 v) {
   if (!lo)
     static_cast<z128*>(p)[i] = v;
@@ -90,14 +102,15 @@ struct PreT {
 };
 using Pre = PreT<kMaxPre>;
 
-template <bool F64, int KP>
+template <bool F64, int KP, bool NT = false>
 __device__ __forceinline__ void prefetch(PreT<KP>& p, const StageArgs& A, int cur, long long gi,
                                          bool ok) {
 #pragma unroll
   for (int s = 0; s < KP; ++s) {
     const void* ptr = cur ? A.pre[1][s] : A.pre[0][s];
     if (ok && s < A.npre && ptr != nullptr)
-      p.v[s] = ldz(ptr, F64 ? false : ((A.pre_lo >> s) & 1u), gi);
+      p.v[s] = NT ? ldz_nt(ptr, F64 ? false : ((A.pre_lo >> s) & 1u), gi)
+                  : ldz(ptr, F64 ? false : ((A.pre_lo >> s) & 1u), gi);
     else
       p.v[s] = z128{0.0, 0.0};
   }
@@ -1072,7 +1085,9 @@ __global__ void __launch_bounds__(BS, BS / 128)
         xs[c] = x < n2 ? x : 0;
         const int i = xs[c] / n, j = xs[c] - (xs[c] / n) * n;
         Dc[c] = C{s_re[i * LD + j], s_im[i * LD + j]};
-        prefetch<F64, KP>(pv[c], A, cur, base + xs[c], true);
+        // FP64 at N = 64: the streamed epilogue operands are loaded non-temporally,
+        // which keeps more of the stage input (the neighbour blocks) cached
+        prefetch<F64, KP, sizeof(R) == 8 && N == 64>(pv[c], A, cur, base + xs[c], true);
       }
       if constexpr (!kLinksOuter) {
         for (int l = l0; l < l1; ++l) {

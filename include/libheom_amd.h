@@ -123,7 +123,9 @@ int heom_set_hrchy_filter(heom_handle h, heom_hrchy_filter filter, void* user);
 
 /* set_param (heom.h:57-139 + heom_hilb.h:57-68 / heom_liou.h:66-119 /
  * heom_ado.h:62-189): builds the hierarchy, neighbour links and coefficient
- * tables and uploads them to HBM. */
+ * tables and uploads them to HBM.  heom_liou / heom_ado take n_level <= 32
+ * (their N^2 x N^2 superoperators grow as N^4) and heom_hilb any n_level with
+ * diagonal V_u, dense V_u up to 64; beyond, HEOM_ERR_UNSUPPORTED. */
 int heom_set_param(heom_handle h);
 int heom_get_n_hrchy(heom_handle h, int* n_hrchy);           /* heom.h:52-55 */
 int heom_get_n_dim(heom_handle h, int* n_dim);
